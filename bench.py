@@ -1,0 +1,206 @@
+"""Benchmark: compressed BAM bytes -> per-record columns on one MI355X.
+
+Workload (default): the ATAC-like paired-end 2 x 50 bp stream of a 200 M-record
+coordinate-sorted file (insert ~ lognormal(median 180, sigma 0.6), 2 % unmapped
+mates, CB:Z from 10,000 barcodes with Zipf(1.1) weights, BGZF zlib level 6,
+65,280-byte payloads), cut into 1 GiB compressed chunks. A step decodes one
+whole chunk already resident in HBM: block scan, inflate + CRC32 check, record
+chain and the column scatter (fixed fields, CIGAR/seq/qual CSR, CB/UB tags).
+
+Prints one JSON line. `value` is inflated GB/s (U / step time); the roofline
+entry is for the inflate kernel, with algorithmic bytes C + U per launch; the
+CPU baseline is the oracle (zlib inflate + CRC check, record walk, columns) on
+all host cores over a bounded sample of the same chunk.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_chunk(chunk_bytes: int, seed: int = 1, n_total: int = 200_000_000):
+    """The first `chunk_bytes` (or a little more) of the config's BGZF file, cut at a block end."""
+    import numpy as np
+
+    from cubam_amd import synth
+
+    # records for about chunk_bytes of compressed data (ratio ~2.2, ~166 bytes a record)
+    n_rec = int(chunk_bytes * 2.25 / 166) + 1000
+    t = time.time()
+    b = synth.make_bam(1, seed, n_rec, n_total=n_total, eof=False)
+    sizes = b.block_sizes.astype(np.int64)
+    ends = np.cumsum(sizes)
+    k = int(np.searchsorted(ends, chunk_bytes, side="right"))
+    k = max(1, min(k, len(sizes)))
+    used = int(ends[k - 1])
+    log(f"generated {b.n_records} records, {b.bgzf.size / 1e9:.2f} GB BGZF in {time.time() - t:.1f} s;"
+        f" chunk = {k} blocks, {used / 2**30:.3f} GiB")
+    return b, b.bgzf[:used], b.header
+
+
+def gpu_step(buf, n, start, n_ref, ws, ev=None):
+    """One pass of the hot path over the chunk in buf[0:n); returns (U, n_rec)."""
+    import torch
+
+    from cubam_amd import _decode
+
+    blocks = _decode.scan_blocks(buf, n, False, ws)
+    if ev is not None:
+        ev[0].record()
+    inf = _decode.inflate_blocks(blocks, None, ws)
+    if ev is not None:
+        ev[1].record()
+    chain = _decode.record_chain(inf.data, start, inf.n, n_ref, ws, inf=inf)
+    off = _decode.record_offsets(inf.data, start, inf.n, chain)
+    cols = _decode.scatter(inf.data, off, chain, tags=("CB", "UB"), ws=ws)
+    return inf, chain, off, cols
+
+
+def column_bytes(chain) -> int:
+    """K: bytes of the column schema written (offset 8 + fixed 35 + CSR + 2 tags x 21 per record)."""
+    n = chain.n_rec
+    return n * (8 + 35 + 3 * 8 + 2 * 21) + 3 * 8 + 4 * chain.n_cigar + chain.n_seq + chain.n_qual
+
+
+def cpu_baseline(chunk, header_size: int, n_ref: int, budget_s: float = 12.0):
+    """The oracle on all host cores over a bounded prefix of the chunk: (GB/s of U, cores, sample)."""
+    import numpy as np
+
+    from oracle import oracle
+
+    oracle.build()
+    cores = os.cpu_count() or 1
+    # a prefix of whole blocks of ~256 MiB compressed (or the chunk)
+    sample = chunk[: min(chunk.size, 256 << 20)]
+    reps, times, U = 0, [], 0
+    t_all = time.time()
+    while reps < 3 and (reps == 0 or time.time() - t_all < budget_s):
+        t = time.perf_counter()
+        off, bs, isz, crc, used, status = oracle.bgzf_scan(sample, False)
+        data, _, got, st = oracle.inflate(sample, off, bs, isz, n_threads=cores)
+        assert status == 0 and (st == 0).all()
+        roff, tail, rst = oracle.record_chain(data[header_size:], n_ref)
+        oracle.columns(data[header_size:], roff, ("CB", "UB"), n_threads=cores)
+        times.append(time.perf_counter() - t)
+        U = int(data.size)
+        reps += 1
+    med = float(np.median(times))
+    return U / med / 1e9, cores, f"first {used / 2**20:.0f} MiB compressed of the chunk ({U / 2**20:.0f} MiB" \
+        f" inflated), median of {reps}"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--chunk-mib", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cubam_amd import _decode, synth
+    from cubam_amd._lib import load
+
+    load()
+    synth.build()
+    torch.cuda.set_device(0)
+    chunk_bytes = args.chunk_mib << 20
+    b, chunk, header = make_chunk(chunk_bytes)
+    hdr = _decode_header(header)
+    n_ref = len(hdr.names)
+    ws = _decode.Workspace()
+    buf = _decode.to_device(chunk)
+    n = int(chunk.size)
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    for _ in range(args.warmup):
+        out = gpu_step(buf, n, hdr.size, n_ref, ws)
+        del out
+    torch.cuda.synchronize()
+    step_ms, inflate_ms = [], []
+    U = K = n_rec = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        inf, chain, off, cols = gpu_step(buf, n, hdr.size, n_ref, ws, ev)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        inflate_ms.append(ev[0].elapsed_time(ev[1]))
+        U, n_rec, K = inf.n, chain.n_rec, column_bytes(chain)
+        del inf, chain, off, cols
+        log(f"step {i}: {step_ms[-1]:.2f} ms (inflate {inflate_ms[-1]:.2f} ms)")
+    total_s = time.perf_counter() - t0
+    ms = total_s * 1e3 / args.steps
+    gbs = U / (ms / 1e3) / 1e9
+    inf_ms = float(np.mean(inflate_ms))
+    achieved = (n + U) / (inf_ms / 1e3) / 1e9
+    res = {
+        "metric": "uncompressed GB/s BAM->columns on 1 GPU (plain per-chunk decode)",
+        "value": round(gbs, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "ATAC-like 2x50bp paired-end stream of a 200M-record BAM, first 1 GiB compressed chunk",
+            "chunk_compressed_bytes": n,
+            "inflated_bytes": U,
+            "records": n_rec,
+            "column_bytes": K,
+            "algorithmic_bytes_C_U_K": n + U + K,
+            "p50_step_ms": round(float(np.percentile(step_ms, 50)), 3),
+            "p99_step_ms": round(float(np.percentile(step_ms, 99)), 3),
+            "path_GBps_C_U_K": round((n + U + K) / (ms / 1e3) / 1e9, 2),
+            "inflate_tune": list(_decode.get_inflate_tune()),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "cbm_inflate (inflate_lanes + crc_blocks)",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "launch_ms": round(inf_ms, 3),
+        },
+    }
+    if not args.no_cpu:
+        v, cores, sample = cpu_baseline(chunk, hdr.size, n_ref)
+        res["cpu_baseline"] = {"value": round(v, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+                               "sample": sample}
+    print(json.dumps(res), flush=True)
+
+
+def _decode_header(raw: bytes):
+    from cubam_amd.bam import BamHeader
+
+    return BamHeader.from_bytes(raw)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,382 @@
+// libcubam_amd: the C ABI declared in include/cubam_amd.h, over the gfx950 kernels of
+// bgzf_scan.h (kernel 1), inflate.h (kernel 2), records.h (kernels 3 and 4).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "bgzf_scan.h"
+#include "common.h"
+#include "inflate.h"
+#include "records.h"
+
+namespace cbm {
+
+static thread_local char g_err[512];
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s: %s", what, hipGetErrorString(e));
+        return CBM_ERR_LAUNCH;
+    }
+    return CBM_OK;
+}
+
+__global__ void __launch_bounds__(1024) scan_tile_bases(int64_t* tile_sum, const int64_t* n_d, int64_t n_max,
+                                                         int64_t* total_out) {
+    __shared__ int64_t sh[1024 / CBM_WAVE];
+    int64_t n = n_d ? *n_d : n_max;
+    int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 0) tiles = 1;
+    // each thread owns a run of consecutive tiles
+    int64_t per = (tiles + 1023) / 1024;
+    int64_t lo = threadIdx.x * per, hi = lo + per < tiles ? lo + per : tiles;
+    int64_t s = 0;
+    for (int64_t t = lo; t < hi; t++) s += tile_sum[t];
+    int64_t tot;
+    int64_t run = block_exclusive_sum<1024, int64_t>(s, sh, &tot);
+    for (int64_t t = lo; t < hi; t++) {
+        int64_t v = tile_sum[t];
+        tile_sum[t] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = tot;
+}
+
+__global__ void fill_i64(int64_t* p, int64_t n, int64_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static int fill(int64_t* p, int64_t n, int64_t v, hipStream_t st) {
+    if (n <= 0) return CBM_OK;
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(fill_i64, dim3((unsigned)g), dim3(256), 0, st, p, n, v);
+    return check_launch("fill");
+}
+
+static int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+// ------------------------------------------------------------------ record chain walk of one segment
+__global__ void rec_walk_one(const uint8_t* b, int64_t start, int64_t n, int n_ref, RecWs w, int64_t k,
+                             int64_t entry) {
+    int64_t lo = start + k * REC_SEG, hi = lo + REC_SEG < n ? lo + REC_SEG : n;
+    (void)lo;
+    WalkOut r = walk(b, n, entry, hi, n_ref);
+    w.f[k] = entry;
+    w.e[k] = r.e;
+    w.cnt[k] = r.cnt;
+    w.cig[k] = r.cig;
+    w.seqb[k] = r.seqb;
+    w.qual[k] = r.qual;
+    w.flag[k] = r.flag;
+    w.pred[k] = 1;
+}
+
+__global__ void rec_zero_dropped(RecWs w) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < w.nseg && w.f[k] < 0) w.cnt[k] = 0;
+}
+
+__global__ void rec_pred_fill(RecWs w) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < w.nseg; k += (int64_t)gridDim.x * blockDim.x)
+        w.pred[k] = w.f[k] >= 0 ? 1 : 0;
+}
+
+}  // namespace cbm
+
+using namespace cbm;
+
+#define ARG(cond, msg)              \
+    do {                            \
+        if (!(cond)) {              \
+            set_error("%s", msg);   \
+            return CBM_ERR_ARG;     \
+        }                           \
+    } while (0)
+
+extern "C" {
+
+const char* cbm_version(void) { return "cubam_amd 0.1 (gfx950)"; }
+
+int cbm_last_error(char* buf, int64_t len) {
+    int n = (int)strlen(g_err);
+    if (buf && len > 0) {
+        int k = n < len - 1 ? n : (int)len - 1;
+        memcpy(buf, g_err, k);
+        buf[k] = 0;
+    }
+    return n;
+}
+
+// ------------------------------------------------------------------ kernel 1
+static int64_t bgzf_max_blocks(int64_t n) { return n / 26 + 2; }
+
+int64_t cbm_bgzf_scan_workspace(int64_t n) {
+    int64_t tiles = (n + BGZF_TILE - 1) / BGZF_TILE + 1;
+    int64_t m = bgzf_max_blocks(n);
+    return 8 * (8 + tiles + 2) + scan_workspace(m > tiles ? m : tiles) + 64;
+}
+
+int cbm_bgzf_scan(const uint8_t* buf, int64_t n, int eof, int64_t cap, int64_t* blk_off, int32_t* bsize,
+                  int32_t* isize, uint32_t* crc, int64_t* out_off, int64_t* result, void* ws, int64_t ws_bytes,
+                  void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(n >= 0 && cap >= 0 && result && ws, "cbm_bgzf_scan: bad arguments");
+    ARG(ws_bytes >= cbm_bgzf_scan_workspace(n), "cbm_bgzf_scan: workspace too small");
+    ARG(n == 0 || (buf && ((uintptr_t)buf & 15) == 0), "cbm_bgzf_scan: buf must be 16-byte aligned");
+    ARG(cap == 0 || (blk_off && bsize && isize && crc && out_off), "cbm_bgzf_scan: null output");
+    if (cap > bgzf_max_blocks(n)) cap = bgzf_max_blocks(n);  // a block has at least 26 bytes
+    int64_t* hdr = (int64_t*)ws;
+    int64_t tiles = (n + BGZF_TILE - 1) / BGZF_TILE;
+    int64_t* tcnt = hdr + 8;
+    void* sws = tcnt + tiles + 2;
+    int rc = fill(hdr, 8, 0, st);
+    if (rc) return rc;
+    if (tiles > 0) {
+        hipLaunchKernelGGL(bgzf_count, dim3((unsigned)tiles), dim3(BGZF_NT), 0, st, buf, n, tcnt);
+        if ((rc = check_launch("bgzf_count"))) return rc;
+        if ((rc = exclusive_scan(LoadI64{tcnt}, StoreI64{tcnt}, (const int64_t*)nullptr, tiles, sws, hdr, st)))
+            return rc;
+        hipLaunchKernelGGL(bgzf_emit, dim3((unsigned)tiles), dim3(BGZF_NT), 0, st, buf, n, tcnt, cap, blk_off);
+        if ((rc = check_launch("bgzf_emit"))) return rc;
+        int64_t g = (cap + 255) / 256;
+        if (g > 1024) g = 1024;
+        if (g < 1) g = 1;
+        hipLaunchKernelGGL(bgzf_links, dim3((unsigned)g), dim3(256), 0, st, buf, blk_off, hdr, cap, hdr + 1);
+        if ((rc = check_launch("bgzf_links"))) return rc;
+    }
+    hipLaunchKernelGGL(bgzf_finish, dim3(1), dim3(1), 0, st, buf, n, eof, cap, blk_off, hdr, result);
+    if ((rc = check_launch("bgzf_finish"))) return rc;
+    if (cap > 0) {
+        int64_t g = (cap + 255) / 256;
+        if (g > 2048) g = 2048;
+        hipLaunchKernelGGL(bgzf_fields, dim3((unsigned)g), dim3(256), 0, st, buf, blk_off, result, bsize, isize,
+                           crc, hdr + 2);
+        if ((rc = check_launch("bgzf_fields"))) return rc;
+        if ((rc = exclusive_scan(LoadI32{isize}, StoreI64{out_off}, result + CBM_SCAN_NBLOCKS, cap, sws,
+                                 result + CBM_SCAN_TOTAL_OUT, st)))
+            return rc;
+    }
+    hipLaunchKernelGGL(bgzf_total, dim3(1), dim3(1), 0, st, hdr + 2, result);
+    return check_launch("bgzf_total");
+}
+
+// ------------------------------------------------------------------ kernel 2
+int64_t cbm_inflate_workspace(int64_t n_blocks) {
+    (void)n_blocks;
+    return 64;
+}
+
+int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst, uint32_t* crc,
+                int32_t* status, int64_t* result, const cbm_inflate_tune* tune, void* ws, int64_t ws_bytes,
+                void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(n_blocks >= 0 && result, "cbm_inflate: bad arguments");
+    (void)ws;
+    (void)ws_bytes;
+    int rc = fill(result, 1, 0, st);
+    if (rc || n_blocks == 0) return rc;
+    ARG(src && blk_off && bsize && isize && crc_expect && out_off && dst && crc && status,
+        "cbm_inflate: null pointer");
+    int bpw = tune && tune->blocks_per_wave > 0 ? tune->blocks_per_wave : 8;
+    int wpg = tune && tune->waves_per_group > 0 ? tune->waves_per_group : 4;
+    ARG(bpw >= 1 && bpw <= 64 && wpg >= 1 && wpg <= 16, "cbm_inflate: bad tune");
+    size_t lds = (sizeof(FixedTables) + 15) / 16 * 16 + (size_t)wpg * bpw * LANE_BYTES;
+    ARG(lds <= 160 * 1024, "cbm_inflate: blocks_per_wave * waves_per_group needs more LDS than a CU has");
+    int64_t per_group = (int64_t)bpw * wpg;
+    int64_t groups = tune && tune->groups > 0 ? tune->groups : (n_blocks + per_group - 1) / per_group;
+    int64_t max_groups = (int64_t)device_cus() * (int64_t)((160 * 1024) / lds);
+    if (groups > max_groups) groups = max_groups;
+    if (groups < 1) groups = 1;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipFuncSetAttribute((const void*)inflate_lanes, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(inflate_lanes, dim3((unsigned)groups), dim3(64 * wpg), lds, st, src, blk_off, bsize, isize,
+                       out_off, n_blocks, dst, status, bpw);
+    if ((rc = check_launch("inflate_lanes"))) return rc;
+    int64_t g = (n_blocks + 3) / 4;
+    if (g > (int64_t)device_cus() * 8) g = (int64_t)device_cus() * 8;
+    hipLaunchKernelGGL(crc_blocks, dim3((unsigned)g), dim3(CRC_NT), 0, st, dst, out_off, isize, crc_expect, n_blocks,
+                       crc, status, result);
+    return check_launch("crc_blocks");
+}
+
+int cbm_crc32_spans(const uint8_t* data, const int64_t* off, const int64_t* len, int64_t n, uint32_t* out,
+                    void* stream) {
+    ARG(n >= 0, "cbm_crc32_spans: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(data && off && len && out, "cbm_crc32_spans: null pointer");
+    int64_t g = (n + 3) / 4;
+    if (g > 65535) g = 65535;
+    hipLaunchKernelGGL(crc_spans, dim3((unsigned)g), dim3(CRC_NT), 0, (hipStream_t)stream, data, off, len, n, out);
+    return check_launch("crc_spans");
+}
+
+// ------------------------------------------------------------------ kernel 3
+int64_t cbm_record_chain_workspace(int64_t n, int64_t* seg_bytes) {
+    if (seg_bytes) *seg_bytes = REC_SEG;
+    return rec_ws_bytes(n);
+}
+
+int cbm_record_chain(const uint8_t* data, int64_t start, int64_t n, int32_t n_ref, int64_t* result, void* ws,
+                     int64_t ws_bytes, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(start >= 0 && n >= start && result && ws, "cbm_record_chain: bad arguments");
+    ARG(ws_bytes >= rec_ws_bytes(n - start), "cbm_record_chain: workspace too small");
+    RecWs w = rec_ws_layout(ws, n - start);
+    hipLaunchKernelGGL(rec_result_init, dim3(1), dim3(64), 0, st, result, start, w);
+    int rc = check_launch("rec_result_init");
+    if (rc || w.nseg == 0) return rc;
+    ARG(data, "cbm_record_chain: null data");
+    unsigned g = (unsigned)((w.nseg + REC_NT - 1) / REC_NT);
+    hipLaunchKernelGGL(rec_walk, dim3(g), dim3(REC_NT), 0, st, data, start, n, n_ref, w);
+    if ((rc = check_launch("rec_walk"))) return rc;
+    hipLaunchKernelGGL(rec_links, dim3(g), dim3(REC_NT), 0, st, start, w);
+    if ((rc = check_launch("rec_links"))) return rc;
+    hipLaunchKernelGGL(rec_sum, dim3(g), dim3(REC_NT), 0, st, w, result);
+    if ((rc = check_launch("rec_sum"))) return rc;
+    hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
+    if ((rc = check_launch("rec_result_links"))) return rc;
+    return exclusive_scan(LoadI64{w.cnt}, StoreI64{w.base}, (const int64_t*)nullptr, w.nseg, w.scan_ws,
+                          (int64_t*)nullptr, st);
+}
+
+int cbm_record_chain_repair(const uint8_t* data, int64_t start, int64_t n, int32_t n_ref, int64_t* result,
+                            void* ws, int64_t ws_bytes, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(start >= 0 && n >= start && result && ws, "cbm_record_chain_repair: bad arguments");
+    ARG(ws_bytes >= rec_ws_bytes(n - start), "cbm_record_chain_repair: workspace too small");
+    RecWs w = rec_ws_layout(ws, n - start);
+    int64_t ns = w.nseg;
+    if (ns == 0) return CBM_OK;
+    std::vector<int64_t> f(ns), e(ns), flag(ns);
+    if (hipMemcpyAsync(f.data(), w.f, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(e.data(), w.e, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(flag.data(), w.flag, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("cbm_record_chain_repair: copy failed");
+        return CBM_ERR_LAUNCH;
+    }
+    std::vector<char> keep(ns, 0);
+    int64_t p = start;
+    while (p < n) {
+        int64_t k = (p - start) / REC_SEG;
+        if (f[k] != p) {
+            hipLaunchKernelGGL(rec_walk_one, dim3(1), dim3(1), 0, st, data, start, n, n_ref, w, k, p);
+            int rc = check_launch("rec_walk_one");
+            if (rc) return rc;
+            if (hipMemcpyAsync(&e[k], w.e + k, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(&flag[k], w.flag + k, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                set_error("cbm_record_chain_repair: copy failed");
+                return CBM_ERR_LAUNCH;
+            }
+            f[k] = p;
+        }
+        keep[k] = 1;
+        if (flag[k]) break;
+        p = e[k];
+    }
+    // walks off the chain no longer count
+    std::vector<int64_t> drop;
+    for (int64_t k = 0; k < ns; k++)
+        if (!keep[k] && f[k] >= 0) drop.push_back(k);
+    for (int64_t k : drop) f[k] = -1;
+    if (hipMemcpyAsync(w.f, f.data(), ns * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("cbm_record_chain_repair: copy failed");
+        return CBM_ERR_LAUNCH;
+    }
+    // counts of dropped walks are masked by f < 0 in rec_sum and rec_emit; recount
+    hipLaunchKernelGGL(rec_result_init, dim3(1), dim3(64), 0, st, result, start, w);
+    hipLaunchKernelGGL(rec_pred_fill, dim3(256), dim3(256), 0, st, w);
+    unsigned g = (unsigned)((ns + REC_NT - 1) / REC_NT);
+    hipLaunchKernelGGL(rec_sum, dim3(g), dim3(REC_NT), 0, st, w, result);
+    int rc = check_launch("rec_sum");
+    if (rc) return rc;
+    // cnt of dropped segments must not enter the scan
+    hipLaunchKernelGGL(rec_zero_dropped, dim3(g), dim3(REC_NT), 0, st, w);
+    if ((rc = check_launch("rec_zero_dropped"))) return rc;
+    hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
+    if ((rc = check_launch("rec_result_links"))) return rc;
+    rc = exclusive_scan(LoadI64{w.cnt}, StoreI64{w.base}, (const int64_t*)nullptr, ns, w.scan_ws,
+                        (int64_t*)nullptr, st);
+    if (rc) return rc;
+    return hipStreamSynchronize(st) == hipSuccess ? CBM_OK : CBM_ERR_LAUNCH;
+}
+
+int cbm_record_offsets(const uint8_t* data, int64_t start, int64_t n, const void* ws, int64_t ws_bytes,
+                       int64_t* offsets, int64_t n_rec, void* stream) {
+    ARG(ws && start >= 0 && n >= start && n_rec >= 0, "cbm_record_offsets: bad arguments");
+    ARG(ws_bytes >= rec_ws_bytes(n - start), "cbm_record_offsets: workspace too small");
+    if (n_rec == 0) return CBM_OK;
+    ARG(data && offsets, "cbm_record_offsets: null pointer");
+    RecWs w = rec_ws_layout((void*)ws, n - start);
+    unsigned g = (unsigned)((w.nseg + REC_NT - 1) / REC_NT);
+    hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, start, n, w, offsets, n_rec);
+    return check_launch("rec_emit");
+}
+
+// ------------------------------------------------------------------ kernel 4
+int64_t cbm_scatter_workspace(int64_t n_rec) { return scan_workspace(n_rec + 1) + 64; }
+
+int cbm_scatter(const uint8_t* data, const int64_t* offsets, int64_t n_rec, const cbm_columns* cols, void* ws,
+                int64_t ws_bytes, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(n_rec >= 0 && cols, "cbm_scatter: bad arguments");
+    ARG(cols->n_tags >= 0 && cols->n_tags <= 4, "cbm_scatter: at most 4 tags");
+    ARG(ws_bytes >= cbm_scatter_workspace(n_rec), "cbm_scatter: workspace too small");
+    int rc;
+    cbm_columns c = *cols;
+    if (n_rec > 0) {
+        ARG(data && offsets, "cbm_scatter: null pointer");
+        unsigned g = (unsigned)((n_rec + SC_NT - 1) / SC_NT);
+        hipLaunchKernelGGL(rec_fields, dim3(g), dim3(SC_NT), 0, st, data, offsets, n_rec, c);
+        if ((rc = check_launch("rec_fields"))) return rc;
+    }
+    if (c.cigar_off &&
+        (rc = exclusive_scan(RecNcig{data, offsets}, StoreI64{c.cigar_off}, (const int64_t*)nullptr, n_rec, ws,
+                             (int64_t*)nullptr, st)))
+        return rc;
+    if (c.seq_off &&
+        (rc = exclusive_scan(RecSeqB{data, offsets}, StoreI64{c.seq_off}, (const int64_t*)nullptr, n_rec, ws,
+                             (int64_t*)nullptr, st)))
+        return rc;
+    if (c.qual_off &&
+        (rc = exclusive_scan(RecQual{data, offsets}, StoreI64{c.qual_off}, (const int64_t*)nullptr, n_rec, ws,
+                             (int64_t*)nullptr, st)))
+        return rc;
+    if (n_rec > 0 && (c.cigar || c.seq || c.qual)) {
+        unsigned g = (unsigned)((n_rec + SC_NT - 1) / SC_NT);
+        hipLaunchKernelGGL(rec_copy_var, dim3(g), dim3(SC_NT), 0, st, data, offsets, n_rec, c);
+        if ((rc = check_launch("rec_copy_var"))) return rc;
+    }
+    return CBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+/*
+ * cubam_amd: compressed BAM bytes -> per-record columns on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libcubam_amd.so. Every pointer argument named *_d is a device
+ * pointer (the caller allocates and frees all of them, e.g. from torch
+ * tensors); `stream` is a hipStream_t. Every entry point only enqueues work on
+ * `stream` and returns at once, except where it says it synchronises.
+ * Return values: CBM_OK (0), or a negative CBM_ERR_* with a message for
+ * cbm_last_error(). Data errors are reported per block / per record in
+ * device memory (status arrays and result slots), never as return codes.
+ *
+ * Input padding: every compressed or inflated byte buffer passed in must be
+ * readable for CBM_PAD bytes past its end (the kernels prefetch ahead).
+ *
+ * The reference (LouisFaure/cubam) has no C ABI of its own; on this path it
+ * binds nvCOMP's batched Deflate C ABI through ctypes
+ * (src/cubam/bgzf.py:161-178, call at :595-611) and launches NVRTC kernels
+ * through CuPy (src/cubam/_kernels.py:637-702). The entry points below replace
+ * those calls one for one, as cited on each.
+ */
+#ifndef CUBAM_AMD_H
+#define CUBAM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBM_PAD 64
+
+enum {
+    CBM_OK = 0,
+    CBM_ERR_ARG = -1,    /* bad argument (null pointer, negative size, small workspace) */
+    CBM_ERR_LAUNCH = -2, /* a HIP launch or runtime call failed */
+};
+
+/* ------------------------------------------------------------------ BGZF block scan (kernel 1) */
+
+/* Result slots written by cbm_bgzf_scan (int64 each, device memory). */
+enum {
+    CBM_SCAN_NBLOCKS = 0,   /* blocks on the chain from offset 0, empty blocks included */
+    CBM_SCAN_USED = 1,      /* bytes of buf those blocks cover; the rest is the carry */
+    CBM_SCAN_STATUS = 2,    /* CBM_SCAN_OK or an error below */
+    CBM_SCAN_TOTAL_OUT = 3, /* sum of isize over the blocks */
+    CBM_SCAN_NCAND = 4,     /* header candidates found (diagnostic) */
+    CBM_SCAN_RESULT_LEN = 8
+};
+enum {
+    CBM_SCAN_OK = 0,
+    CBM_SCAN_TRUNCATED = 1, /* eof and the chain does not end at n, or no block at all */
+    CBM_SCAN_NO_BLOCK = 2,  /* no block header at offset 0 though there are others */
+    CBM_SCAN_CORRUPT = 3,   /* isize > 65536, or an empty block that is not the EOF block */
+    CBM_SCAN_CAP = 4        /* more blocks than cap */
+};
+
+/* Bytes of workspace cbm_bgzf_scan needs for a buffer of n bytes. */
+int64_t cbm_bgzf_scan_workspace(int64_t n);
+
+/*
+ * Find the BGZF blocks of buf_d[0:n) (which starts at a block): header
+ * candidates (the fixed gzip/BGZF header fields, BSIZE+1 >= 26, block inside
+ * the buffer), the chain from offset 0, then validation. Writes, per block on
+ * the chain (at most cap): blk_off, bsize, isize, the trailer CRC32 (crc) and
+ * the output offset out_off (exclusive prefix sum of isize; cap + 1 entries,
+ * out_off[n_blocks] = total); and the result slots above. buf_d must be
+ * 16-byte aligned. Replaces append_offsets("bgzf_candidates") + bgzf_sizes +
+ * chain + the checks of src/cubam/bgzf.py:549-584 (kernels at
+ * src/cubam/_kernels.py:69-89, 662-702).
+ */
+int cbm_bgzf_scan(const uint8_t* buf_d, int64_t n, int eof, int64_t cap, int64_t* blk_off_d, int32_t* bsize_d,
+                  int32_t* isize_d, uint32_t* crc_d, int64_t* out_off_d, int64_t* result_d, void* ws_d,
+                  int64_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------ inflate + CRC32 (kernel 2) */
+
+/* Per-block status written by cbm_inflate. */
+enum {
+    CBM_BLOCK_OK = 0,
+    CBM_BLOCK_DEFLATE = 1, /* invalid DEFLATE stream (bad block type, code, distance, overrun) */
+    CBM_BLOCK_SIZE = 2,    /* it does not inflate to isize bytes */
+    CBM_BLOCK_CRC = 3      /* CRC32 of the output differs from the trailer */
+};
+
+/* Launch shape of the inflate kernel; zeros select the defaults (or the autotuned choice). */
+typedef struct {
+    int32_t blocks_per_wave; /* lanes of a wavefront that each inflate a block: 1..64 */
+    int32_t waves_per_group; /* wavefronts per workgroup: 1..16 */
+    int32_t layout;          /* LDS decode-table layout: 0 lane-major, 1 lane-interleaved */
+    int32_t groups;          /* workgroups launched (persistent); 0 = enough for all blocks */
+} cbm_inflate_tune;
+
+/* Bytes of workspace cbm_inflate needs for n_blocks blocks. */
+int64_t cbm_inflate_workspace(int64_t n_blocks);
+
+/*
+ * Inflate n_blocks raw-DEFLATE BGZF payloads (src_d + blk_off[i] + 18, bsize[i]
+ * - 26 bytes) into dst_d + out_off[i] (isize[i] bytes each), then check each
+ * output's CRC32 against crc_expect[i] with one wavefront per block. Writes
+ * crc_d[i] (CRC32 of what was written) and status_d[i]; result_d[0] receives
+ * the number of blocks whose status is not CBM_BLOCK_OK.
+ * Replaces nvcompBatchedDeflateDecompressAsync (src/cubam/bgzf.py:595-611)
+ * and adds the CRC32 check the reference skips (docs/reading.md:48-51), using
+ * the combination method of crc32_spans (src/cubam/_kernels.py:93-141).
+ */
+int cbm_inflate(const uint8_t* src_d, const int64_t* blk_off_d, const int32_t* bsize_d, const int32_t* isize_d,
+                const uint32_t* crc_expect_d, const int64_t* out_off_d, int64_t n_blocks, uint8_t* dst_d,
+                uint32_t* crc_d, int32_t* status_d, int64_t* result_d, const cbm_inflate_tune* tune, void* ws_d,
+                int64_t ws_bytes, void* stream);
+
+/* CRC32 (zlib) of n spans data_d[off[i] : off[i] + len[i]], one wavefront per span. Replaces
+   crc32_spans (src/cubam/_kernels.py:106-141). */
+int cbm_crc32_spans(const uint8_t* data_d, const int64_t* off_d, const int64_t* len_d, int64_t n, uint32_t* out_d,
+                    void* stream);
+
+/* ------------------------------------------------------------------ record chain (kernel 3) */
+
+/* Result slots written by cbm_record_chain (int64 each, device memory). */
+enum {
+    CBM_REC_N = 0,         /* complete records */
+    CBM_REC_TAIL = 1,      /* offset (from data start) of the incomplete last record: the carry */
+    CBM_REC_INVALID = 2,   /* records that fail the structural test (is_record) */
+    CBM_REC_CIGAR = 3,     /* sum of n_cigar_op */
+    CBM_REC_SEQ = 4,       /* sum of ceil(l_seq / 2) */
+    CBM_REC_QUAL = 5,      /* sum of l_seq */
+    CBM_REC_BAD_LINKS = 6, /* segments whose speculative walks did not chain (host repairs them) */
+    CBM_REC_NSEG = 7,      /* segments */
+    CBM_REC_RESULT_LEN = 8
+};
+
+/* Bytes of workspace for n inflated bytes; *seg_bytes receives the segment size used. */
+int64_t cbm_record_chain_workspace(int64_t n, int64_t* seg_bytes);
+
+/*
+ * Find the record chain of data_d[start : n) (start is a record boundary) by
+ * walking block_size from speculative per-segment starts and checking that the
+ * walks link up. Writes the result slots and keeps per-segment state in ws_d
+ * for cbm_record_offsets. Replaces append_offsets("record_candidates") +
+ * record_ends + chain of src/cubam/bam.py:403-418 (kernels at
+ * src/cubam/_kernels.py:250-289, 676-702).
+ */
+int cbm_record_chain(const uint8_t* data_d, int64_t start, int64_t n, int32_t n_ref, int64_t* result_d, void* ws_d,
+                     int64_t ws_bytes, void* stream);
+
+/*
+ * Repair segments whose walks did not link (result slot CBM_REC_BAD_LINKS > 0):
+ * walks the chain serially from start and rewrites the per-segment state.
+ * Synchronises the stream. Rare: only data where the structural test finds a
+ * false record start, or corrupt data.
+ */
+int cbm_record_chain_repair(const uint8_t* data_d, int64_t start, int64_t n, int32_t n_ref, int64_t* result_d,
+                            void* ws_d, int64_t ws_bytes, void* stream);
+
+/* Offsets (from data start) of the n_rec records found by cbm_record_chain, in order. */
+int cbm_record_offsets(const uint8_t* data_d, int64_t start, int64_t n, const void* ws_d, int64_t ws_bytes,
+                       int64_t* offsets_d, int64_t n_rec, void* stream);
+
+/* ------------------------------------------------------------------ column scatter (kernel 4) */
+
+/* Column outputs of cbm_scatter (device pointers; a NULL pointer skips that column). */
+typedef struct {
+    int32_t *ref, *pos, *end;
+    uint8_t* mapq;
+    uint16_t* flag;
+    int32_t *nref, *npos, *tlen, *l_seq;
+    uint16_t *clip_left, *clip_right;
+    int64_t* cigar_off; /* [n + 1] */
+    uint32_t* cigar;    /* raw CIGAR words (len << 4 | op) */
+    int64_t* seq_off;   /* [n + 1] */
+    uint8_t* seq;       /* 4-bit packed bases, BAM order (high nibble first) */
+    int64_t* qual_off;  /* [n + 1] */
+    uint8_t* qual;
+    /* aux tags: up to 4 names, each type u8 / value offset i64 / length i32 / value i64 per record */
+    int32_t n_tags;
+    char tag_names[8];
+    uint8_t* tag_type[4];
+    int64_t* tag_offset[4];
+    int32_t* tag_length[4];
+    int64_t* tag_value[4];
+} cbm_columns;
+
+/* Bytes of workspace cbm_scatter needs for n_rec records. */
+int64_t cbm_scatter_workspace(int64_t n_rec);
+
+/*
+ * Decode the records at offsets_d[0:n_rec) of data_d into columns: the 11
+ * fixed fields of decode_fixed (src/cubam/_kernels.py:292-319), the CSR
+ * CIGAR / seq / qual columns, and aux tags as find_tag
+ * (src/cubam/_kernels.py:334-388). Replaces RecordBatch.decode
+ * (src/cubam/bam.py:268-288) and Tag.__init__ (src/cubam/bam.py:184-208).
+ */
+int cbm_scatter(const uint8_t* data_d, const int64_t* offsets_d, int64_t n_rec, const cbm_columns* cols, void* ws_d,
+                int64_t ws_bytes, void* stream);
+
+/* ------------------------------------------------------------------ misc */
+
+/* Copy the last error message of this thread into buf (NUL-terminated); returns its length. */
+int cbm_last_error(char* buf, int64_t len);
+
+/* Library version string. */
+const char* cbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBAM_AMD_H */

@@ -1,0 +1,331 @@
+"""The HIP path against the oracle and the test writer, bit for bit (needs a GPU).
+
+Every comparison is exact: integer columns, inflated bytes, per-block CRC32.
+"""
+
+import gzip
+import random
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import bamfixture as bf
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+@pytest.fixture(scope="module")
+def cbm():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import cubam_amd
+    from cubam_amd import _lib
+
+    _lib.load()
+    return cubam_amd
+
+
+def _hetero(n, seed, max_size=65280):
+    from cubam_amd import synth
+
+    return synth.hetero_blocks(n, seed, max_size=max_size)
+
+
+def _gpu_blocks(blob, tune=None):
+    import torch
+
+    from cubam_amd import _decode
+
+    buf = _decode.to_device(blob)
+    blocks = _decode.scan_blocks(buf, int(blob.size), True)
+    inf = _decode.inflate_blocks(blocks, None, tune=tune)
+    torch.cuda.synchronize()
+    return blocks, inf
+
+
+def test_hetero_blocks_bytes_and_crc(cbm, oracle):
+    hb = _hetero(3000, 2)
+    _say("hetero blocks", hb.bgzf.size, "bytes", len(hb.block_off), "blocks")
+    blocks, inf = _gpu_blocks(hb.bgzf)
+    assert blocks.n_blocks == len(hb.block_off)
+    assert blocks.blk_off.cpu().numpy().tolist() == hb.block_off.tolist()
+    assert blocks.isize.cpu().numpy().tolist() == hb.raw_size.tolist()
+    assert int(inf.result.item()) == 0
+    assert (inf.status.cpu().numpy() == 0).all()
+    got = inf.data[: inf.n].cpu().numpy()
+    assert got.tobytes() == hb.raw.tobytes()
+    crc = inf.crc.cpu().numpy().astype(np.int64)
+    expect = [zlib.crc32(hb.raw[o : o + s].tobytes()) for o, s in zip(hb.raw_off, hb.raw_size)]
+    assert crc.tolist() == expect
+    _, _, ocrc, ost = oracle.inflate(hb.bgzf, hb.block_off, hb.block_size, hb.raw_size)
+    assert (ost == 0).all() and ocrc.astype(np.int64).tolist() == expect
+
+
+@pytest.mark.parametrize("bpw,wpg", [(1, 1), (2, 4), (4, 2), (8, 4), (16, 3), (32, 1), (64, 1)])
+def test_inflate_every_launch_shape(cbm, bpw, wpg):
+    from cubam_amd import _lib
+
+    hb = _hetero(700, 3)
+    _say("shape", bpw, wpg)
+    tune = _lib.InflateTune(bpw, wpg, 0, 0)
+    blocks, inf = _gpu_blocks(hb.bgzf, tune)
+    assert int(inf.result.item()) == 0
+    assert inf.data[: inf.n].cpu().numpy().tobytes() == hb.raw.tobytes()
+
+
+def test_tiny_and_empty_blocks(cbm):
+    # an empty block must be the 28-byte EOF block (src/cubam/bgzf.py:572), so b"" only at level 6
+    payloads = [b"x", b"ab" * 3, bytes(range(256)), b"\0" * 35000, b"A" * 65280]
+    blob = bf.bgzf_block(b"", 6) + b"".join(bf.bgzf_block(p, lvl) for p in payloads for lvl in (0, 1, 6, 9))
+    blob += bf.EOF_BLOCK
+    blocks, inf = _gpu_blocks(np.frombuffer(blob, np.uint8))
+    assert int(inf.result.item()) == 0
+    assert inf.data[: inf.n].cpu().numpy().tobytes() == b"".join(p for p in payloads for _ in range(4))
+
+
+def _data(n, seed=0):
+    rng = random.Random(seed)
+    words = [bytes(rng.choices(b"ACGT\t\n0123456789", k=rng.randint(1, 30))) for _ in range(500)]
+    out = bytearray()
+    while len(out) < n:
+        out += rng.choice(words)
+    return bytes(out[:n])
+
+
+def _read_chunks(path, **kw):
+    from cubam_amd import inflate_chunks
+
+    return b"".join(c.cpu().numpy().tobytes() for c in inflate_chunks(path, **kw))
+
+
+@pytest.mark.parametrize(("block", "chunk"), [(1000, 4096), (65280, 1 << 20), (300, 777)])
+def test_inflate_chunks_roundtrip(cbm, tmp_path, block, chunk):
+    data = _data(300_000)
+    p = tmp_path / "x.gz"
+    p.write_bytes(bf.bgzf(data, block))
+    assert _read_chunks(p, chunk_bytes=chunk, piece=chunk // 3 + 1) == data
+
+
+def test_chunks_are_on_the_gpu(cbm, tmp_path):
+    from cubam_amd import inflate_chunks
+
+    p = tmp_path / "x.gz"
+    p.write_bytes(bf.bgzf(_data(10_000), 1000))
+    assert all(c.is_cuda for c in inflate_chunks(p, chunk_bytes=4096))
+
+
+def test_empty_file(cbm, tmp_path):
+    p = tmp_path / "e.gz"
+    p.write_bytes(bf.EOF_BLOCK)
+    assert _read_chunks(p, chunk_bytes=4096) == b""
+
+
+def test_truncated(cbm, tmp_path):
+    p = tmp_path / "t.gz"
+    p.write_bytes(bf.bgzf(_data(50_000), 1000, eof=False)[:-100])
+    with pytest.raises(ValueError, match="Truncated"):
+        _read_chunks(p, chunk_bytes=8192)
+
+
+@pytest.mark.parametrize("where", ["btype", "isize", "crc", "payload"])
+def test_corrupt_block(cbm, tmp_path, where):
+    raw = bytearray(bf.bgzf(_data(50_000), 1000))
+    bsize = raw[16] + (raw[17] << 8) + 1
+    if where == "btype":
+        raw[18] |= 0b110  # reserved Deflate block type
+    elif where == "isize":
+        raw[bsize - 4] ^= 0x01
+    elif where == "crc":
+        raw[bsize - 8] ^= 0x01
+    else:
+        raw[40] ^= 0x55
+    p = tmp_path / "c.gz"
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ValueError):
+        _read_chunks(p, chunk_bytes=8192)
+
+
+def test_not_bgzf(cbm, tmp_path):
+    p = tmp_path / "n.gz"
+    p.write_bytes(b"hello world, not compressed" * 100)
+    with pytest.raises(ValueError):
+        _read_chunks(p, chunk_bytes=4096)
+
+
+def _gpu_read(path, **kw):
+    from cubam_amd import read_bam
+
+    return list(read_bam(path, **kw))
+
+
+def _cat(batches, k):
+    import torch
+
+    return torch.cat([getattr(b, k).cpu() for b in batches]).numpy() if batches else np.empty(0)
+
+
+@pytest.mark.parametrize(("block", "chunk"), [(4096, 1 << 20), (700, 5000), (65280, 64 << 10)])
+def test_read_bam_fields(cbm, tmp_path, block, chunk):
+    reads = bf.random_reads(3000)
+    p = tmp_path / "r.bam"
+    p.write_bytes(bf.bgzf(bf.bam_bytes(reads), block))
+    batches = _gpu_read(p, chunk_bytes=chunk, piece=chunk // 2 + 1, tags=("CB", "NM", "RG"))
+    assert sum(len(b) for b in batches) == len(reads)
+    for k in ("ref", "pos", "end", "mapq", "flag", "nref", "npos", "tlen", "l_seq", "clip_left", "clip_right"):
+        got = _cat(batches, k).astype(np.int64).tolist()
+        exp = [getattr(r, k) if hasattr(r, k) else None for r in reads]
+        if k == "l_seq":
+            exp = [len(r.seq) for r in reads]
+        elif k in ("clip_left", "clip_right"):
+            exp = [r.clips()[k == "clip_right"] for r in reads]
+        assert got == exp, k
+    cb, rg, nm, seqs, quals, cigs = [], [], [], [], [], []
+    for b in batches:
+        cb += b.tag("CB").to_list()
+        rg += b.tag("RG").to_list()
+        t = b.tag("NM")
+        nm += np.where(t.present.cpu().numpy(), t.value.cpu().numpy(), -1).tolist()
+        so, s = b.seq_off.cpu().numpy(), b.seq.cpu().numpy()
+        qo, q = b.qual_off.cpu().numpy(), b.qual.cpu().numpy()
+        co, c = b.cigar_off.cpu().numpy(), b.cigar.cpu().numpy()
+        for i in range(len(b)):
+            seqs.append(s[so[i] : so[i + 1]].tobytes())
+            quals.append(q[qo[i] : qo[i + 1]].tobytes())
+            cigs.append(c[co[i] : co[i + 1]].astype(np.int64).tolist())
+    assert cb == [(r.tag("CB") or ("Z", ""))[1] for r in reads]
+    assert rg == [r.tag("RG")[1] for r in reads]
+    assert nm == [r.tag("NM")[1] for r in reads]
+    assert cigs == [[(n << 4) | bf.OPS.index(op) for op, n in r.cigar] for r in reads]
+    for r, s, q in zip(reads, seqs, quals):
+        codes = [bf.SEQ.index(ch) for ch in r.seq] + [0]
+        assert s == bytes((codes[i] << 4) | codes[i + 1] for i in range(0, len(r.seq), 2))
+        assert q == (r.qual if r.qual is not None else b"\xff" * len(r.seq))
+
+
+def test_five_prime_take_and_flags(cbm, tmp_path):
+    reads = bf.random_reads(500, seed=3)
+    p = tmp_path / "t.bam"
+    p.write_bytes(bf.bgzf(bf.bam_bytes(reads), 4096))
+    (b,) = _gpu_read(p, tags=("NM",))
+    five = b.five_prime().cpu().numpy().tolist()
+    exp = [r.end + r.clips()[1] if r.flag & 16 else r.pos - r.clips()[0] for r in reads]
+    assert five == exp
+    sel = b.has_flags(all_of=1, none_of=0x400)
+    sub = b.take(sel)
+    ex = [r for r in reads if r.flag & 1 and not r.flag & 0x400]
+    assert len(sub) == len(ex)
+    assert sub.pos.cpu().numpy().tolist() == [r.pos for r in ex]
+    assert sub.tag("NM").value.cpu().numpy().tolist() == [r.tag("NM")[1] for r in ex]
+    assert sub.data.is_cuda
+
+
+def test_long_reads_cross_blocks(cbm, tmp_path):
+    reads = [bf.Read(f"long{i}", 0, 0, 1000 * i, 60, [("M", 20000)]) for i in range(20)]
+    p = tmp_path / "l.bam"
+    p.write_bytes(bf.bgzf(bf.bam_bytes(reads), 4096))
+    got = [int(x) for b in _gpu_read(p, chunk_bytes=30_000) for x in b.end.cpu()]
+    assert got == [r.end for r in reads]
+
+
+def test_header_only(cbm, tmp_path):
+    from cubam_amd import BamHeader
+
+    text = "@HD\tVN:1.6\tSO:coordinate\n@SQ\tSN:chr1\tLN:1000000\n@PG\tID:bwa\tPN:bwa\n"
+    p = tmp_path / "h.bam"
+    p.write_bytes(bf.bgzf(bf.bam_bytes([], text=text), 4096))
+    h = BamHeader.read(p)
+    assert h.names == [n for n, _ in bf.REFS]
+    assert h.sort_order == "coordinate"
+    assert h.lines("@PG") == ["@PG\tID:bwa\tPN:bwa"]
+    assert _gpu_read(p) == []
+
+
+def test_truncated_record(cbm, tmp_path):
+    reads = bf.random_reads(50, seed=4)
+    p = tmp_path / "tr.bam"
+    p.write_bytes(bf.bgzf(bf.bam_bytes(reads)[:-10], 4096))
+    with pytest.raises(ValueError, match="truncated BAM"):
+        _gpu_read(p)
+
+
+def _compare_with_oracle(batches, obatches, tags=("CB", "UB")):
+    import torch
+
+    keys = ("ref", "pos", "end", "mapq", "flag", "nref", "npos", "tlen", "l_seq", "clip_left", "clip_right")
+    for k in keys:
+        g = np.concatenate([getattr(b, k).cpu().numpy() for b in batches])
+        o = np.concatenate([c[k] for _, c in obatches])
+        assert np.array_equal(g.astype(np.int64), o.astype(np.int64)), k
+    for (b, (d, c)) in zip(batches, obatches):
+        assert torch.equal(b.offset.cpu(), torch.from_numpy(c["offset"]))
+        for k in ("cigar_off", "seq_off", "qual_off", "seq", "qual"):
+            assert np.array_equal(getattr(b, k).cpu().numpy(), c[k]), k
+        assert np.array_equal(b.cigar.cpu().numpy().astype(np.uint32), c["cigar"])
+        assert b.data[: d.size].cpu().numpy().tobytes() == d.tobytes()
+        for t in tags:
+            tg = b.tag(t)
+            oc = c[f"tag_{t}"]
+            for f in ("type", "offset", "length", "value"):
+                assert np.array_equal(getattr(tg, f).cpu().numpy(), oc[f]), (t, f)
+
+
+@pytest.mark.parametrize("kind, n, chunk", [(0, 20_000, 1 << 20), (1, 30_000, 300_000), (1, 5000, 1 << 26)])
+def test_synthetic_bam_matches_oracle(cbm, oracle, synth, tmp_path, kind, n, chunk):
+    b = synth.make_bam(kind, 11 + kind, n, n_total=50_000_000)
+    p = b.write(tmp_path / "s.bam")
+    _say("synthetic", kind, n, chunk, b.bgzf.size)
+    batches = _gpu_read(p, chunk_bytes=chunk)
+    obatches = oracle.read_bam(b.bgzf, chunk)
+    assert len(batches) == len(obatches)
+    assert sum(len(x) for x in batches) == n
+    _compare_with_oracle(batches, obatches)
+
+
+def test_record_chain_repair_is_identity(cbm, oracle, synth):
+    """The serial repair walk gives the same chain as the speculative walks."""
+    import torch
+
+    from cubam_amd import _decode, _lib
+
+    b = synth.make_bam(1, 21, 4000, n_total=10_000_000)
+    raw = np.frombuffer(gzip.decompress(b.bgzf.tobytes()), np.uint8)
+    data = _decode.to_device(raw)
+    start, n = len(b.header), int(raw.size)
+    chain = _decode.record_chain(data, start, n, 25)
+    off = _decode.record_offsets(data, start, n, chain).cpu().numpy()
+    L = _lib.load()
+    res = torch.zeros(8, dtype=torch.int64, device=data.device)
+    _lib.check(L.cbm_record_chain_repair(data.data_ptr(), start, n, 25, res.data_ptr(), chain.ws.data_ptr(),
+                                         chain.ws.numel(), _lib.stream_ptr()), "repair")
+    r = res.cpu().numpy()
+    assert r[_lib.REC_N] == chain.n_rec == 4000 and r[_lib.REC_TAIL] == n and r[_lib.REC_BAD_LINKS] == 0
+    off2 = _decode.record_offsets(data, start, n, chain).cpu().numpy()
+    ooff, tail, st = oracle.record_chain(raw[start:], 25)
+    assert off.tolist() == off2.tolist() == (ooff + start).tolist()
+
+
+def test_crc32_spans_matches_zlib(cbm):
+    import torch
+
+    from cubam_amd import _decode, _lib
+
+    rng = np.random.default_rng(3)
+    raw = rng.integers(0, 256, 500_000, dtype=np.uint8)
+    d = _decode.to_device(raw)
+    off = np.arange(0, raw.size, 30_001)
+    size = np.minimum(off + 30_001, raw.size) - off
+    off_d = torch.from_numpy(off.astype(np.int64)).cuda()
+    len_d = torch.from_numpy(size.astype(np.int64)).cuda()
+    out = torch.empty(off.size, dtype=torch.uint32, device="cuda")
+    _lib.check(_lib.load().cbm_crc32_spans(d.data_ptr(), off_d.data_ptr(), len_d.data_ptr(), off.size,
+                                           out.data_ptr(), _lib.stream_ptr()), "crc")
+    got = out.cpu().numpy().astype(np.int64).tolist()
+    assert got == [zlib.crc32(raw[o : o + s].tobytes()) for o, s in zip(off, size)]
