@@ -29,7 +29,7 @@ FIXED = ("ref", "pos", "end", "mapq", "flag", "nref", "npos", "tlen", "l_seq", "
 
 
 def _strings(vals):
-    b = [v.encode() for v in vals]
+    b = [v if isinstance(v, bytes) else v.encode() for v in vals]
     off = np.zeros(len(b) + 1, np.int64)
     np.cumsum([len(x) for x in b], out=off[1:])
     return np.frombuffer(b"".join(b), np.uint8).copy(), off
@@ -46,7 +46,7 @@ def bam_fixture(name: str, kind: int, seed: int, n: int):
     out = {k: np.asarray([r[k] for r in recs], np.int64) for k in FIXED}
     out["offset"] = np.asarray([r["offset"] for r in recs], np.int64) - len(b.header)
     for k in ("seq", "qual"):
-        out[k], out[k + "_off"] = _strings([r[k].decode("latin1") for r in recs])
+        out[k], out[k + "_off"] = _strings([r[k] for r in recs])
     cig = [np.asarray(r["cigar"], np.uint32) for r in recs]
     out["cigar"] = np.concatenate(cig)
     out["cigar_off"] = np.concatenate([[0], np.cumsum([c.size for c in cig])]).astype(np.int64)
@@ -72,7 +72,7 @@ def blocks_fixture():
         blocks.append(bf.bgzf_block(p, lvl, strat))
         levels.append(lvl)
     (HERE / "blocks.bgzf").write_bytes(b"".join(blocks) + bf.EOF_BLOCK)
-    pl, off = _strings([p.decode("latin1") for p in payloads])
+    pl, off = _strings(payloads)
     np.savez_compressed(HERE / "blocks_expect.npz", payload=pl, payload_off=off,
                         crc=np.asarray([zlib.crc32(p) for p in payloads], np.int64),
                         level=np.asarray(levels, np.int64))
