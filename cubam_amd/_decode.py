@@ -75,7 +75,7 @@ def to_device(host_u8, dev: torch.device | None = None) -> torch.Tensor:
     if isinstance(host_u8, (bytes, bytearray, memoryview)):
         host_u8 = np.frombuffer(bytes(host_u8), np.uint8)
     if not isinstance(host_u8, torch.Tensor):
-        host_u8 = torch.from_numpy(np.ascontiguousarray(host_u8, np.uint8))
+        host_u8 = torch.from_numpy(np.require(host_u8, np.uint8, ["C", "W"]))
     n = host_u8.numel()
     out = torch.empty(padded(n), dtype=_U8, device=dev or device())
     out[:n].copy_(host_u8, non_blocking=False)
