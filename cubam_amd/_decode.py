@@ -142,14 +142,14 @@ def scan_blocks(buf: torch.Tensor, n: int, eof: bool, ws: Workspace | None = Non
 _tune = _lib.InflateTune(0, 0, 0, 0)
 
 
-def set_inflate_tune(blocks_per_wave: int = 0, waves_per_group: int = 0, layout: int = 0, groups: int = 0) -> None:
+def set_inflate_tune(variant: int = 0, blocks_per_wave: int = 0, waves_per_group: int = 0, groups: int = 0) -> None:
     """Launch shape of the inflate kernel for this process (zeros: library defaults)."""
     global _tune
-    _tune = _lib.InflateTune(blocks_per_wave, waves_per_group, layout, groups)
+    _tune = _lib.InflateTune(variant, blocks_per_wave, waves_per_group, groups)
 
 
 def get_inflate_tune() -> tuple[int, int, int, int]:
-    return (_tune.blocks_per_wave, _tune.waves_per_group, _tune.layout, _tune.groups)
+    return (_tune.variant, _tune.blocks_per_wave, _tune.waves_per_group, _tune.groups)
 
 
 @dataclass
@@ -177,11 +177,12 @@ def inflate_blocks(blocks: Blocks, prefix: torch.Tensor | None = None, ws: Works
     status = torch.empty(max(nb, 1), dtype=_I32, device=dev)
     result = torch.empty(1, dtype=_I64, device=dev)
     t = tune if tune is not None else _tune
+    iws = ws.dev("inflate_ws", L.cbm_inflate_workspace(nb))
     check(
         L.cbm_inflate(
             ptr(blocks.buf), ptr(blocks.blk_off), ptr(blocks.bsize), ptr(blocks.isize), ptr(blocks.crc),
             ptr(blocks.out_off), nb, data.data_ptr() + p, ptr(crc), ptr(status), ptr(result), ctypes.byref(t),
-            None, 0, stream_ptr(),
+            ptr(iws), iws.numel(), stream_ptr(),
         ),
         "cbm_inflate",
     )
@@ -190,7 +191,7 @@ def inflate_blocks(blocks: Blocks, prefix: torch.Tensor | None = None, ws: Works
 
 def raise_block_errors(inf: Inflated, bad: int) -> None:
     if bad:
-        st = inf.status.cpu()
+        st = inf.status.cpu() & 0xFF  # bits 8-15 name the check that failed
         if bool(((st == _lib.BLOCK_DEFLATE) | (st == _lib.BLOCK_SIZE)).any()):
             raise ValueError(MSG_SIZE)
         raise ValueError(MSG_CRC)

@@ -22,11 +22,14 @@ BLOCK_OK, BLOCK_DEFLATE, BLOCK_SIZE, BLOCK_CRC = 0, 1, 2, 3
 REC_N, REC_TAIL, REC_INVALID, REC_CIGAR, REC_SEQ, REC_QUAL, REC_BAD_LINKS, REC_NSEG = range(8)
 
 
+INFLATE_WG256, INFLATE_LANES, INFLATE_WG128, INFLATE_WG512, INFLATE_WG64 = 0, 1, 2, 3, 4
+
+
 class InflateTune(ctypes.Structure):
     _fields_ = [
+        ("variant", ctypes.c_int32),
         ("blocks_per_wave", ctypes.c_int32),
         ("waves_per_group", ctypes.c_int32),
-        ("layout", ctypes.c_int32),
         ("groups", ctypes.c_int32),
     ]
 

@@ -10,6 +10,7 @@
 #include "bgzf_scan.h"
 #include "common.h"
 #include "inflate.h"
+#include "inflate_wave.h"
 #include "records.h"
 
 namespace cbm {
@@ -104,6 +105,33 @@ __global__ void rec_pred_fill(RecWs w) {
         w.pred[k] = w.f[k] >= 0 ? 1 : 0;
 }
 
+// workgroups of the persistent inflate launch: the LDS of a CU holds two block buffers
+static int64_t inflate_max_groups() { return (int64_t)device_cus() * 2; }
+
+template <int NT, int SEG>
+static int launch_wave(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                       const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                       uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t* counter,
+                       hipStream_t st) {
+    const size_t lds = sizeof(WaveSmem<NT, SEG>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)inflate_wave<NT, SEG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        attr = true;
+    }
+    const size_t per_cu = (160 * 1024) / (lds + 1024);
+    int64_t g = groups > 0 ? groups : (int64_t)device_cus() * (int64_t)(per_cu ? per_cu : 1);
+    if (g > inflate_max_groups()) g = inflate_max_groups();  // the workspace holds that many match lists
+    if (g > n_blocks) g = n_blocks;
+    if (g < 1) g = 1;
+    uint64_t* mlists = (uint64_t*)((uint8_t*)counter + 256);
+    hipLaunchKernelGGL((inflate_wave<NT, SEG>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
+                       crc_expect, out_off, n_blocks, dst, crc, status, result, (unsigned long long*)counter, mlists,
+                       (unsigned long long*)counter + 8);
+    return check_launch("inflate_wave");
+}
+
 }  // namespace cbm
 
 using namespace cbm;
@@ -186,7 +214,7 @@ int cbm_bgzf_scan(const uint8_t* buf, int64_t n, int eof, int64_t cap, int64_t* 
 // ------------------------------------------------------------------ kernel 2
 int64_t cbm_inflate_workspace(int64_t n_blocks) {
     (void)n_blocks;
-    return 64;
+    return 256 + inflate_max_groups() * (int64_t)MLIST_CAP * 8;
 }
 
 int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
@@ -195,34 +223,58 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
                 void* stream) {
     hipStream_t st = (hipStream_t)stream;
     ARG(n_blocks >= 0 && result, "cbm_inflate: bad arguments");
-    (void)ws;
-    (void)ws_bytes;
     int rc = fill(result, 1, 0, st);
     if (rc || n_blocks == 0) return rc;
     ARG(src && blk_off && bsize && isize && crc_expect && out_off && dst && crc && status,
         "cbm_inflate: null pointer");
-    int bpw = tune && tune->blocks_per_wave > 0 ? tune->blocks_per_wave : 8;
-    int wpg = tune && tune->waves_per_group > 0 ? tune->waves_per_group : 4;
+    const int variant = tune ? tune->variant : 0;
+    const int64_t groups = tune ? tune->groups : 0;
+    if (variant != CBM_INFLATE_LANES) {
+        ARG(ws && ws_bytes >= cbm_inflate_workspace(n_blocks), "cbm_inflate: workspace too small");
+        int64_t* counter = (int64_t*)ws;  // [0] next block, [8..15] decoder statistics
+        if ((rc = fill(counter, 16, 0, st))) return rc;
+        switch (variant) {
+            case CBM_INFLATE_WG256: return launch_wave<256, 128>(src, blk_off, bsize, isize, crc_expect, out_off,
+                                                                 n_blocks, dst, crc, status, result, groups, counter, st);
+            case CBM_INFLATE_WG128: return launch_wave<128, 256>(src, blk_off, bsize, isize, crc_expect, out_off,
+                                                                 n_blocks, dst, crc, status, result, groups, counter, st);
+            case CBM_INFLATE_WG512: return launch_wave<512, 64>(src, blk_off, bsize, isize, crc_expect, out_off,
+                                                                n_blocks, dst, crc, status, result, groups, counter, st);
+            case CBM_INFLATE_WG64: return launch_wave<64, 512>(src, blk_off, bsize, isize, crc_expect, out_off,
+                                                               n_blocks, dst, crc, status, result, groups, counter, st);
+            case 5: return launch_wave<256, 256>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                 status, result, groups, counter, st);
+            case 6: return launch_wave<128, 128>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                 status, result, groups, counter, st);
+            case 7: return launch_wave<64, 128>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                status, result, groups, counter, st);
+            case 8: return launch_wave<64, 256>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                status, result, groups, counter, st);
+            default: ARG(false, "cbm_inflate: unknown variant");
+        }
+    }
+    int bpw = tune && tune->blocks_per_wave > 0 ? tune->blocks_per_wave : 4;
+    int wpg = tune && tune->waves_per_group > 0 ? tune->waves_per_group : 2;
     ARG(bpw >= 1 && bpw <= 64 && wpg >= 1 && wpg <= 16, "cbm_inflate: bad tune");
     size_t lds = (sizeof(FixedTables) + 15) / 16 * 16 + (size_t)wpg * bpw * LANE_BYTES;
     ARG(lds <= 160 * 1024, "cbm_inflate: blocks_per_wave * waves_per_group needs more LDS than a CU has");
     int64_t per_group = (int64_t)bpw * wpg;
-    int64_t groups = tune && tune->groups > 0 ? tune->groups : (n_blocks + per_group - 1) / per_group;
+    int64_t g = groups > 0 ? groups : (n_blocks + per_group - 1) / per_group;
     int64_t max_groups = (int64_t)device_cus() * (int64_t)((160 * 1024) / lds);
-    if (groups > max_groups) groups = max_groups;
-    if (groups < 1) groups = 1;
+    if (g > max_groups) g = max_groups;
+    if (g < 1) g = 1;
     static bool attr_set = false;
     if (!attr_set) {
-        hipFuncSetAttribute((const void*)inflate_lanes, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)inflate_lanes, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(inflate_lanes, dim3((unsigned)groups), dim3(64 * wpg), lds, st, src, blk_off, bsize, isize,
+    hipLaunchKernelGGL(inflate_lanes, dim3((unsigned)g), dim3(64 * wpg), lds, st, src, blk_off, bsize, isize,
                        out_off, n_blocks, dst, status, bpw);
     if ((rc = check_launch("inflate_lanes"))) return rc;
-    int64_t g = (n_blocks + 3) / 4;
-    if (g > (int64_t)device_cus() * 8) g = (int64_t)device_cus() * 8;
-    hipLaunchKernelGGL(crc_blocks, dim3((unsigned)g), dim3(CRC_NT), 0, st, dst, out_off, isize, crc_expect, n_blocks,
-                       crc, status, result);
+    int64_t cg = (n_blocks + 3) / 4;
+    if (cg > (int64_t)device_cus() * 8) cg = (int64_t)device_cus() * 8;
+    hipLaunchKernelGGL(crc_blocks, dim3((unsigned)cg), dim3(CRC_NT), 0, st, dst, out_off, isize, crc_expect,
+                       n_blocks, crc, status, result);
     return check_launch("crc_blocks");
 }
 

@@ -1,0 +1,714 @@
+// Kernel 2, workgroup variant: one BGZF block per workgroup, decoded by all its
+// threads at once, output staged in LDS, CRC32 fused.
+//
+// DEFLATE is serial within a Huffman block, but Huffman codes resynchronise: a
+// decoder started at an arbitrary bit soon falls onto the true symbol boundaries.
+// Each round of a Huffman block gives thread t the SEG bits that start at
+// base + t * SEG. Pass A decodes speculatively from there to the first symbol that
+// starts at or after the next segment, and remembers every symbol start it saw.
+// The walks are then linked: thread t's true entry is thread t-1's exit; a thread
+// whose entry changed decodes again from it until it lands on a start it already
+// saw (from there its first walk was right) — repeated until nothing changes.
+// Pass B counts each thread's output bytes, a workgroup prefix sum places them,
+// and pass C writes literals and copies matches in LDS; a match whose source lies
+// in a segment still being written waits on that segment's progress counter
+// (the earliest waiting match always has its source written, so this ends).
+// The block's output then leaves LDS in one coalesced store, and its CRC32 is
+// computed from LDS by all threads (slices combined with x^(8n) mod P).
+//
+// Global memory is touched only by the coalesced staging of each round's
+// compressed bits into LDS and by the final store: no load ever waits behind the
+// decoder's own stores.
+#pragma once
+#include "inflate.h"
+
+namespace cbm {
+
+template <int NT, int SEG>
+struct WaveSmem {
+    static constexpr int IN_WORDS = NT * SEG / 32 + 8;  // staged compressed bits of a round
+    uint8_t out[65536 + 16];
+    uint32_t in[IN_WORDS];
+    uint16_t lit[LTAB];
+    uint16_t dist[DTAB];
+    uint16_t code[320];
+    uint8_t lens[320];
+    int32_t exit_[NT];
+    int32_t stop_[NT];
+    int32_t red[NT / CBM_WAVE];
+    int64_t red64[NT / CBM_WAVE];
+    int32_t lcount[16];
+    int32_t lnext[16];
+    int32_t chunkcnt[5][16];
+    int32_t sv[16];  // shared scalars (SV_*)
+};
+
+enum { SV_POS = 0, SV_FINAL, SV_TYPE, SV_ERR, SV_LEN, SV_SRC, SV_OP, SV_CLEAR, SV_TEND, SV_CHANGED, SV_DONE, SV_OK,
+       SV_BLOCK, SV_MAX, SV_SITE };
+
+// the check that failed, reported in bits 8-15 of a block's status
+enum { SITE_HEADER = 1, SITE_STORED = 2, SITE_TYPE = 3, SITE_CLCODE = 4, SITE_LENGTHS = 5, SITE_TABLES = 6,
+       SITE_ROUND = 7, SITE_DIST = 8, SITE_OVERRUN = 9, SITE_SIZE = 10, SITE_ROUND_SIZE = 11 };
+
+enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
+
+// decoder statistics (cycles of thread 0 per phase, and rounds), summed over blocks
+enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
+       STAT_N };
+
+// matches of a round, per workgroup, in global scratch: a block of at most 65,536 bytes has at
+// most 21,846 matches of 3 bytes or more
+constexpr int MLIST_CAP = 21888;
+
+// length / distance bases and extra bits of RFC 1951 3.2.5, computed
+__device__ __forceinline__ void len_code(int c, int& base, int& extra) {
+    if (c < 8) {
+        base = c + 3;
+        extra = 0;
+    } else if (c < 28) {
+        extra = (c >> 2) - 1;
+        base = ((4 + (c & 3)) << extra) + 3;
+    } else {
+        base = 258;
+        extra = 0;
+    }
+}
+__device__ __forceinline__ void dist_code(int d, int& base, int& extra) {
+    if (d < 4) {
+        base = d + 1;
+        extra = 0;
+    } else {
+        extra = (d >> 1) - 1;
+        base = ((2 + (d & 1)) << extra) + 1;
+    }
+}
+
+// 64 bits of the staged stream from bit p (p relative to the payload start)
+template <class S>
+__device__ __forceinline__ uint64_t peek64(const S& s, int p, int wbase) {
+    int w = (p >> 5) - wbase;
+    uint32_t lo = s.in[w], mid = s.in[w + 1], hi = s.in[w + 2];
+    int sh = p & 31;
+    uint64_t v = ((uint64_t)mid << 32) | lo;
+    v >>= sh;
+    if (sh) v |= (uint64_t)hi << (64 - sh);
+    return v;
+}
+
+struct Sym {
+    int kind;  // 0 literal, 1 match, 2 end of block, 3 invalid
+    int bits;  // bits used
+    int val;   // literal byte or match length
+    int dist;
+};
+
+template <class S>
+__device__ __forceinline__ Sym decode_at(const S& s, int p, int wbase) {
+    Sym r;
+    uint64_t v = peek64(s, p, wbase);
+    uint32_t e = s.lit[v & ((1u << LBITS) - 1)];
+    int used;
+    if (e & 0x8000) {
+        int m = e & 15, off = (e >> 4) & 0x7ff;
+        uint32_t e2 = s.lit[off + ((uint32_t)(v >> LBITS) & ((1u << m) - 1))];
+        used = LBITS + (e2 & 15);
+        e = (e2 & 15) ? e2 : 0;
+    } else {
+        used = e & 15;
+    }
+    if (!(e & 15)) {
+        r.kind = 3;
+        r.bits = 1;
+        return r;
+    }
+    int sym = (e >> 4) & 0x1ff;
+    if (sym < 256) {
+        r.kind = 0;
+        r.bits = used;
+        r.val = sym;
+        return r;
+    }
+    if (sym == 256) {
+        r.kind = 2;
+        r.bits = used;
+        return r;
+    }
+    if (sym > 285) {
+        r.kind = 3;
+        r.bits = used;
+        return r;
+    }
+    int lb, le;
+    len_code(sym - 257, lb, le);
+    v >>= used;
+    int len = lb + (int)(v & ((1u << le) - 1));
+    v >>= le;
+    used += le;
+    uint32_t d = s.dist[v & ((1u << DBITS) - 1)];
+    int dused;
+    if (d & 0x8000) {
+        int m = d & 15, off = (d >> 4) & 0x7ff;
+        uint32_t d2 = s.dist[off + ((uint32_t)(v >> DBITS) & ((1u << m) - 1))];
+        dused = DBITS + (d2 & 15);
+        d = (d2 & 15) ? d2 : 0;
+    } else {
+        dused = d & 15;
+    }
+    int ds = (d >> 4) & 0x1ff;
+    if (!(d & 15) || ds >= 30) {
+        r.kind = 3;
+        r.bits = used + (dused ? dused : 1);
+        return r;
+    }
+    v >>= dused;
+    used += dused;
+    int db, de;
+    dist_code(ds, db, de);
+    r.kind = 1;
+    r.val = len;
+    r.dist = db + (int)(v & ((1u << de) - 1));
+    r.bits = used + de;
+    return r;
+}
+
+// ------------------------------------------------------------------ cooperative table build
+// Canonical Huffman decode table of lens[0:nsym) into tab (root bits, cap entries), with the
+// checks of build_table; scratch32 holds 1 << root words. Returns (in all threads) success.
+template <int NT, class S>
+__device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, uint16_t* tab, int cap, int type,
+                           uint32_t* scratch32) {
+    const int tid = threadIdx.x, lane = tid % CBM_WAVE, wave = tid / CBM_WAVE;
+    const int rsize = 1 << root;
+    if (tid < 16) s.lcount[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < nsym; i += NT) atomicAdd(&s.lcount[lens[i]], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int max = 15;
+        while (max >= 1 && s.lcount[max] == 0) max--;
+        int ok = 1, left = 1;
+        for (int l = 1; l < 16; l++) {
+            left = (left << 1) - s.lcount[l];
+            if (left < 0) ok = 0;
+        }
+        if (ok && max > 0 && left > 0 && (type == TAB_CODES || max != 1)) ok = 0;
+        int code = 0;
+        s.lcount[0] = 0;
+        for (int l = 1; l < 16; l++) {
+            code = (code + s.lcount[l - 1]) << 1;
+            s.lnext[l] = code;
+        }
+        s.sv[SV_OK] = ok;
+        s.sv[SV_MAX] = max;
+        s.sv[SV_CLEAR] = (left > 0 || max > root || max == 0) ? 1 : 0;  // root needs clearing
+    }
+    __syncthreads();
+    if (!s.sv[SV_OK]) return false;
+    const int max = s.sv[SV_MAX];
+    if (s.sv[SV_CLEAR])
+        for (int i = tid; i < rsize; i += NT) tab[i] = 0;
+    // canonical codes: rank of each symbol among symbols of its length, by 64-symbol chunks
+    const int nchunk = (nsym + CBM_WAVE - 1) / CBM_WAVE;
+    for (int c = wave; c < nchunk; c += NT / CBM_WAVE) {
+        int sidx = c * CBM_WAVE + lane;
+        int L = sidx < nsym ? lens[sidx] : 0;
+        int rank = 0;
+        for (int l = 1; l <= 15; l++) {
+            uint64_t m = __ballot(L == l);
+            if (L == l) rank = __popcll(m & ((1ull << lane) - 1));
+            if (lane == 0) s.chunkcnt[c][l] = __popcll(m);
+        }
+        if (sidx < nsym) s.code[sidx] = (uint16_t)rank;
+    }
+    __syncthreads();
+    for (int i = tid; i < nsym; i += NT) {
+        int L = lens[i];
+        if (!L) continue;
+        int c = i / CBM_WAVE, base = s.lnext[L];
+        for (int k = 0; k < c; k++) base += s.chunkcnt[k][L];
+        s.code[i] = (uint16_t)(base + s.code[i]);
+    }
+    __syncthreads();
+    if (max > root) {
+        // deepest code under each root prefix (atomicMax on 32-bit scratch), then place subtables
+        for (int i = tid; i < rsize; i += NT) scratch32[i] = 0;
+        __syncthreads();
+        for (int i = tid; i < nsym; i += NT) {
+            int L = lens[i];
+            if (L > root) atomicMax(&scratch32[bitrev(s.code[i], L) & (rsize - 1)], (uint32_t)(L - root));
+        }
+        __syncthreads();
+        // exclusive prefix of subtable sizes over prefixes (each thread owns consecutive prefixes)
+        const int per = (rsize + NT - 1) / NT;
+        int lo = tid * per, hi = lo + per < rsize ? lo + per : rsize;
+        int sz = 0;
+        for (int i = lo; i < hi; i++) sz += scratch32[i] ? (1 << scratch32[i]) : 0;
+        int tot;
+        int off = rsize + block_exclusive_sum<NT, int>(sz, s.red, &tot);
+        if (rsize + tot > cap) return false;  // uniform: every thread sees the same total
+        for (int i = lo; i < hi; i++) {
+            int m = scratch32[i];
+            if (m) {
+                tab[i] = (uint16_t)(0x8000 | (off << 4) | m);
+                off += 1 << m;
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < nsym; i += NT) {
+        int L = lens[i];
+        if (!L) continue;
+        uint32_t rev = bitrev(s.code[i], L);
+        if (L <= root) {
+            uint16_t e = (uint16_t)((i << 4) | L);
+            for (uint32_t j = rev; j < (uint32_t)rsize; j += 1u << L) tab[j] = e;
+        } else {
+            uint16_t p = tab[rev & (rsize - 1)];
+            int m = p & 15, off = (p >> 4) & 0x7ff, sl = L - root;
+            uint16_t e = (uint16_t)((i << 4) | sl);
+            for (uint32_t j = rev >> root; j < (1u << m); j += 1u << sl) tab[off + j] = e;
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// ------------------------------------------------------------------ staging
+// Stage payload words [w0, w0 + IN_WORDS) (word = 4 payload bytes; payload byte 0 = in[0])
+// into LDS; bytes at or past `limit` (payload bytes + trailer) read as zero.
+template <int NT, class S>
+__device__ void stage(S& s, const uint8_t* in, int w0, int limit) {
+    uintptr_t a = (uintptr_t)in;
+    const uint32_t* al = (const uint32_t*)(a & ~(uintptr_t)3);
+    int sh = (int)(a & 3);
+    for (int i = threadIdx.x; i < S::IN_WORDS; i += NT) {
+        int byte0 = 4 * (w0 + i);
+        uint32_t w = 0;
+        if (byte0 + 4 <= limit) {
+            int k = (byte0 + sh) >> 2;
+            uint32_t x0 = al[k];
+            w = sh ? ((x0 >> (8 * sh)) | (al[k + 1] << (32 - 8 * sh))) : x0;
+        } else if (byte0 < limit) {
+            for (int j = 0; j < 4 && byte0 + j < limit; j++) w |= (uint32_t)in[byte0 + j] << (8 * j);
+        }
+        s.in[i] = w;
+    }
+}
+
+// ------------------------------------------------------------------ the kernel
+template <int NT, int SEG>
+__global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int64_t* blk_off,
+                                                   const int32_t* bsize, const int32_t* isize,
+                                                   const uint32_t* expect, const int64_t* out_off,
+                                                   int64_t n_blocks, uint8_t* dst, uint32_t* crc_out,
+                                                   int32_t* status, int64_t* bad_count,
+                                                   unsigned long long* next_block, uint64_t* mlists,
+                                                   unsigned long long* stats) {
+    using S = WaveSmem<NT, SEG>;
+    extern __shared__ __align__(16) uint8_t smem_raw[];
+    S& s = *(S*)smem_raw;
+    __shared__ uint32_t crcT[256];
+    const int tid = threadIdx.x;
+    uint64_t* mlist = mlists + (size_t)blockIdx.x * MLIST_CAP;
+    // cycles per phase, kept by thread 0 and added to stats[] once per block
+    long long acc[STAT_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long tlast = tid == 0 ? clock64() : 0;
+#define PHASE(k)                          \
+    do {                                  \
+        if (tid == 0) {                   \
+            long long t_ = clock64();     \
+            acc[k] += t_ - tlast;         \
+            tlast = t_;                   \
+        }                                 \
+    } while (0)
+    // CRC tables and this thread's slice factor x^(8 * slice * (NT - 1 - tid)) mod P
+    constexpr int SLICE = 65536 / NT;
+    for (int i = tid; i < 256; i += NT) {
+        uint32_t c = i;
+        for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+        crcT[i] = c;
+    }
+    uint32_t xfac;
+    {
+        uint32_t r = 0x80000000u, sq = 0x40000000u;  // x^0, x^1
+        for (int j = 0; j < 3; j++) sq = crc_mulmod(sq, sq);  // x^8
+        uint32_t k = (uint32_t)SLICE * (uint32_t)(NT - 1 - tid);
+        while (k) {
+            if (k & 1) r = crc_mulmod(r, sq);
+            sq = crc_mulmod(sq, sq);
+            k >>= 1;
+        }
+        xfac = r;
+    }
+    __syncthreads();
+
+    for (;;) {
+        if (tid == 0) s.sv[SV_BLOCK] = (int)atomicAdd(next_block, 1ull);
+        __syncthreads();
+        const int64_t b = s.sv[SV_BLOCK];
+        if (b >= n_blocks) break;
+        const uint8_t* in = src + blk_off[b] + 18;
+        const int in_bytes = bsize[b] - 26;
+        const int in_bits = 8 * in_bytes;
+        const int limit = in_bytes + 8;
+        const int cap = isize[b];
+        if (tid == 0) {
+            s.sv[SV_POS] = 0;
+            s.sv[SV_OP] = 0;
+            s.sv[SV_ERR] = CBM_BLOCK_OK;
+            s.sv[SV_SITE] = 0;
+            s.sv[SV_FINAL] = 0;
+        }
+        __syncthreads();
+        while (!s.sv[SV_FINAL] && s.sv[SV_ERR] == CBM_BLOCK_OK) {
+            // ---- block header, read by thread 0 from a fresh staging at pos
+            const int pos0 = s.sv[SV_POS];
+            const int wb0 = pos0 >> 5;
+            stage<NT>(s, in, wb0, limit);
+            __syncthreads();
+            if (tid == 0) {
+                int p = pos0;
+                if (p + 3 > in_bits) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_HEADER;
+                uint64_t v = peek64(s, p, wb0);
+                s.sv[SV_FINAL] = (int)(v & 1);
+                int type = (int)((v >> 1) & 3);
+                s.sv[SV_TYPE] = type;
+                p += 3;
+                if (type == 0) {
+                    p = (p + 7) & ~7;
+                    uint64_t w = peek64(s, p, wb0);
+                    uint32_t len = (uint32_t)(w & 0xffff), nlen = (uint32_t)((w >> 16) & 0xffff);
+                    p += 32;
+                    if (len != (~nlen & 0xffffu) || (p >> 3) + (int)len > in_bytes)
+                        s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_STORED;
+                    else if (s.sv[SV_OP] + (int)len > cap)
+                        s.sv[SV_ERR] = CBM_BLOCK_SIZE, s.sv[SV_SITE] = SITE_STORED;
+                    s.sv[SV_LEN] = (int)len;
+                    s.sv[SV_SRC] = p >> 3;
+                    p += 8 * (int)len;
+                } else if (type == 2) {
+                    uint32_t hlit = (uint32_t)((v >> 3) & 31) + 257, hdist = (uint32_t)((v >> 8) & 31) + 1;
+                    uint32_t hclen = (uint32_t)((v >> 13) & 15) + 4;
+                    p += 14;
+                    if (hlit > 286 || hdist > 30) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_HEADER;
+                    for (int i = 0; i < 19; i++) s.lens[i] = 0;
+                    for (uint32_t i = 0; i < hclen; i++) {
+                        s.lens[kClOrder[i]] = (uint8_t)(peek64(s, p, wb0) & 7);
+                        p += 3;
+                    }
+                    s.sv[SV_LEN] = (int)hlit;
+                    s.sv[SV_SRC] = (int)hdist;
+                } else if (type == 3) {
+                    s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_TYPE;
+                }
+                s.sv[SV_POS] = p;
+            }
+            __syncthreads();
+            if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
+            const int type = s.sv[SV_TYPE];
+            if (type == 0) {
+                const int len = s.sv[SV_LEN], sp = s.sv[SV_SRC], op = s.sv[SV_OP];
+                for (int i = tid; i < len; i += NT) s.out[op + i] = in[sp + i];
+                __syncthreads();
+                if (tid == 0) s.sv[SV_OP] = op + len;
+                __syncthreads();
+                continue;
+            }
+            if (type == 1) {
+                for (int i = tid; i < 288; i += NT) s.lens[i] = i < 144 ? 8 : i < 256 ? 9 : i < 280 ? 7 : 8;
+                __syncthreads();
+                coop_build<NT>(s, s.lens, 288, LBITS, s.lit, LTAB, TAB_LENS, s.in);
+                for (int i = tid; i < 32; i += NT) s.lens[i] = 5;
+                __syncthreads();
+                coop_build<NT>(s, s.lens, 32, DBITS, s.dist, DTAB, TAB_DISTS, s.in);
+            } else {
+                const int hlit = s.sv[SV_LEN], hdist = s.sv[SV_SRC];
+                // the code-length code, then the code lengths (serial, thread 0); the
+                // code-length table goes in the distance table's space
+                bool ok = coop_build<NT>(s, s.lens, 19, CBITS, s.dist, DTAB, TAB_CODES, s.in);
+                if (!ok) {
+                    if (tid == 0) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_CLCODE;
+                    __syncthreads();
+                    break;
+                }
+                const int p1 = s.sv[SV_POS];
+                const int wb1 = p1 >> 5;
+                stage<NT>(s, in, wb1, limit);
+                __syncthreads();
+                if (tid == 0) {
+                    int p = p1, n = 0, total = hlit + hdist, err = 0;
+                    int wb = wb1;
+                    while (n < total) {
+                        if ((p >> 5) - wb + 3 >= S::IN_WORDS) break;  // needs a new staging (rare)
+                        uint64_t v = peek64(s, p, wb);
+                        uint32_t e = s.dist[v & 127];
+                        int l = e & 15, sym = (e >> 4) & 0x1ff;
+                        if (!l) { err = 1; break; }
+                        p += l;
+                        v >>= l;
+                        if (sym < 16) {
+                            s.lens[n++] = (uint8_t)sym;
+                        } else {
+                            int rep;
+                            uint8_t val = 0;
+                            if (sym == 16) {
+                                if (n == 0) { err = 1; break; }
+                                val = s.lens[n - 1];
+                                rep = 3 + (int)(v & 3);
+                                p += 2;
+                            } else if (sym == 17) {
+                                rep = 3 + (int)(v & 7);
+                                p += 3;
+                            } else {
+                                rep = 11 + (int)(v & 127);
+                                p += 7;
+                            }
+                            if (n + rep > total) { err = 1; break; }
+                            while (rep--) s.lens[n++] = val;
+                        }
+                        if (p > in_bits) { err = 1; break; }
+                    }
+                    if (!err && n < total) err = 2;  // ran past the staged window
+                    if (!err && s.lens[256] == 0) err = 1;
+                    s.sv[SV_ERR] = err == 1 ? CBM_BLOCK_DEFLATE : CBM_BLOCK_OK;
+                    if (err) s.sv[SV_SITE] = SITE_LENGTHS;
+                    s.sv[SV_CHANGED] = err == 2;
+                    s.sv[SV_POS] = p;
+                }
+                __syncthreads();
+                if (s.sv[SV_CHANGED]) {
+                    // code lengths longer than a staging window: cannot happen for 320 lengths of
+                    // at most 7 + 7 bits (4,480 bits) with windows of NT * SEG >= 8,192 bits
+                    if (tid == 0) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_LENGTHS;
+                    __syncthreads();
+                }
+                if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
+                ok = coop_build<NT>(s, s.lens, hlit, LBITS, s.lit, LTAB, TAB_LENS, s.in);
+                if (ok) {
+                    // distance lengths follow the literal lengths
+                    for (int i = tid; i < hdist; i += NT) s.code[i] = s.lens[hlit + i];
+                    __syncthreads();
+                    for (int i = tid; i < hdist; i += NT) s.lens[i] = (uint8_t)s.code[i];
+                    __syncthreads();
+                    ok = coop_build<NT>(s, s.lens, hdist, DBITS, s.dist, DTAB, TAB_DISTS, s.in);
+                }
+                if (!ok) {
+                    if (tid == 0) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_TABLES;
+                    __syncthreads();
+                    break;
+                }
+            }
+            PHASE(STAT_HEADER);
+            // ---- Huffman-coded data, round by round
+            for (;;) {
+                const int base = s.sv[SV_POS];
+                const int wb = base >> 5;
+                stage<NT>(s, in, wb, limit);
+                __syncthreads();
+                const int a = base + tid * SEG;         // speculative start
+                const int aend = a + SEG;
+                const bool active = a < in_bits;        // thread 0 is always active
+                // pass A
+                uint32_t seen[SEG / 32];
+#pragma unroll
+                for (int k = 0; k < SEG / 32; k++) seen[k] = 0;
+                int p = a, stop = STOP_NONE;
+                if (active) {
+                    while (p < aend) {
+                        int r = p - a;
+#pragma unroll
+                        for (int k = 0; k < SEG / 32; k++)
+                            if ((r >> 5) == k) seen[k] |= 1u << (r & 31);
+                        Sym y = decode_at(s, p, wb);
+                        p += y.bits;
+                        if (y.kind == 2) { stop = STOP_EOB; break; }
+                        if (y.kind == 3 || p > in_bits) { stop = STOP_BAD; break; }
+                    }
+                }
+                s.exit_[tid] = p;
+                s.stop_[tid] = active ? stop : STOP_BAD;
+                int entry = a;
+                // a later walk that lands on a start of this first walk takes this walk's end
+                const int a_exit = p, a_stop = stop;
+                __syncthreads();
+                PHASE(STAT_PASS_A);
+                // link the walks
+                for (;;) {
+                    int want = tid == 0 ? base : s.exit_[tid - 1];
+                    int pstop = tid == 0 ? STOP_NONE : s.stop_[tid - 1];
+                    if (tid == 0 && threadIdx.x == 0) s.sv[SV_CHANGED] = 0;
+                    __syncthreads();
+                    bool redo = active && pstop == STOP_NONE && want != entry;
+                    int np = p, nstop = stop;
+                    if (redo) {
+                        np = want;
+                        nstop = STOP_NONE;
+                        bool synced = false;
+                        while (np < aend) {
+                            int r = np - a;
+                            if (r >= 0) {
+                                uint32_t bit = 0;
+#pragma unroll
+                                for (int k = 0; k < SEG / 32; k++)
+                                    if ((r >> 5) == k) bit = (seen[k] >> (r & 31)) & 1;
+                                if (bit) { synced = true; break; }
+                            }
+                            Sym y = decode_at(s, np, wb);
+                            np += y.bits;
+                            if (y.kind == 2) { nstop = STOP_EOB; break; }
+                            if (y.kind == 3 || np > in_bits) { nstop = STOP_BAD; break; }
+                        }
+                        if (synced) {
+                            np = a_exit;
+                            nstop = a_stop;
+                        }
+                        entry = want;
+                        if (np != p || nstop != stop) s.sv[SV_CHANGED] = 1;
+                        p = np;
+                        stop = nstop;
+                        s.exit_[tid] = p;
+                        s.stop_[tid] = stop;
+                    }
+                    __syncthreads();
+                    int changed = s.sv[SV_CHANGED];
+                    __syncthreads();
+                    if (!changed) break;
+                }
+                PHASE(STAT_LINK);
+                // the first walk that stops ends the round (and the Huffman block, on EOB)
+                if (tid == 0) s.sv[SV_TEND] = NT;
+                __syncthreads();
+                if (stop != STOP_NONE) atomicMin(&s.sv[SV_TEND], tid);
+                __syncthreads();
+                const int tend = s.sv[SV_TEND];
+                const bool live = tid <= tend && tid < NT;
+                // pass B: output bytes and matches of each live walk, entry .. exit
+                int cnt = 0, nmatch = 0;
+                if (live) {
+                    int q = tid == 0 ? base : s.exit_[tid - 1];
+                    const int e = s.exit_[tid];
+                    while (q < e) {
+                        Sym y = decode_at(s, q, wb);
+                        q += y.bits;
+                        if (y.kind == 0) cnt += 1;
+                        else if (y.kind == 1) cnt += y.val, nmatch++;
+                        else break;
+                    }
+                }
+                int64_t tot2;
+                const int64_t pre = block_exclusive_sum<NT, int64_t>(((int64_t)nmatch << 32) | (uint32_t)cnt,
+                                                                      (int64_t*)s.red64, &tot2);
+                const int ob = (int)(uint32_t)pre, mbase = (int)(pre >> 32);
+                const int total = (int)(uint32_t)tot2, mtotal = (int)(tot2 >> 32);
+                const int op = s.sv[SV_OP];
+                const int tstop = tend < NT ? s.stop_[tend] : STOP_NONE;
+                if (tstop == STOP_BAD || op + total > cap) {
+                    __syncthreads();
+                    if (tid == 0) {
+                        s.sv[SV_ERR] = tstop == STOP_BAD ? CBM_BLOCK_DEFLATE : CBM_BLOCK_SIZE;
+                        s.sv[SV_SITE] = tstop == STOP_BAD ? SITE_ROUND : SITE_ROUND_SIZE;
+                    }
+                    __syncthreads();
+                    break;
+                }
+                PHASE(STAT_PASS_B);
+                // pass C: literals go to their place; matches go to the list, in output order
+                if (live && cnt) {
+                    int q = tid == 0 ? base : s.exit_[tid - 1];
+                    const int e = s.exit_[tid];
+                    int w = op + ob, k = mbase;
+                    bool bad = false;
+                    while (q < e) {
+                        Sym y = decode_at(s, q, wb);
+                        q += y.bits;
+                        if (y.kind == 0) {
+                            s.out[w++] = (uint8_t)y.val;
+                        } else if (y.kind == 1) {
+                            if (y.dist > w) { bad = true; break; }
+                            mlist[k++] = (uint64_t)w | ((uint64_t)y.dist << 17) | ((uint64_t)y.val << 33);
+                            w += y.val;
+                        } else {
+                            break;
+                        }
+                    }
+                    if (bad) {
+                        atomicOr(&s.sv[SV_ERR], CBM_BLOCK_DEFLATE);
+                        s.sv[SV_SITE] = SITE_DIST;
+                    }
+                }
+                __threadfence_block();
+                __syncthreads();
+                if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
+                PHASE(STAT_PASS_C);
+                // pass D: one wavefront applies the matches in order, 64 lanes per copy; every
+                // source byte precedes its match, so it is final when the match is reached
+                if (tid < CBM_WAVE) {
+                    const int lane = tid;
+                    for (int k0 = 0; k0 < mtotal; k0 += CBM_WAVE) {
+                        uint64_t mine = k0 + lane < mtotal ? __atomic_load_n(&mlist[k0 + lane], __ATOMIC_RELAXED) : 0;
+                        const int nb = mtotal - k0 < CBM_WAVE ? mtotal - k0 : CBM_WAVE;
+                        for (int j = 0; j < nb; j++) {
+                            const uint64_t m = __shfl(mine, j, CBM_WAVE);
+                            const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+                            const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
+                            if (dist >= len) {
+                                for (int i = lane; i < len; i += CBM_WAVE) s.out[dest + i] = s.out[src + i];
+                            } else {
+                                for (int i = lane; i < len; i += CBM_WAVE) s.out[dest + i] = s.out[src + i % dist];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                PHASE(STAT_PASS_D);
+                if (tid == 0) acc[STAT_ROUNDS]++;
+                if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
+                if (tid == 0) {
+                    s.sv[SV_OP] = op + total;
+                    s.sv[SV_POS] = tend < NT ? s.exit_[tend] : s.exit_[NT - 1];
+                }
+                __syncthreads();
+                if (tend < NT) break;  // end of the Huffman block
+            }
+        }
+        // ---- block end: size, store, CRC32
+        __syncthreads();
+        PHASE(STAT_HEADER);
+        int err = s.sv[SV_ERR], site = s.sv[SV_SITE];
+        const int op = s.sv[SV_OP];
+        if (err == CBM_BLOCK_OK && s.sv[SV_POS] > in_bits) err = CBM_BLOCK_DEFLATE, site = SITE_OVERRUN;
+        if (err == CBM_BLOCK_OK && op != cap) err = CBM_BLOCK_SIZE, site = SITE_SIZE;
+        const int nout = err == CBM_BLOCK_OK ? cap : (op < cap ? op : cap);
+        uint8_t* d = dst + out_off[b];
+        for (int i = tid; i < nout; i += NT) d[i] = s.out[i];
+        // CRC32 of the output: slice tid covers [cap - SLICE*(NT - tid), cap - SLICE*(NT-1-tid))
+        uint32_t c = 0;
+        {
+            int e2 = nout - SLICE * (NT - 1 - tid), s2 = e2 - SLICE;
+            if (s2 < 0) s2 = 0;
+            if (e2 > s2) {
+                uint32_t x = 0xFFFFFFFFu;
+                for (int k = s2; k < e2; k++) x = crcT[(x ^ s.out[k]) & 255] ^ (x >> 8);
+                c = crc_mulmod(xfac, x ^ 0xFFFFFFFFu);
+            }
+        }
+        for (int dd = 32; dd; dd >>= 1) c ^= __shfl_xor(c, dd, CBM_WAVE);
+        if ((tid % CBM_WAVE) == 0) s.red[tid / CBM_WAVE] = (int)c;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t x = 0;
+            for (int k = 0; k < NT / CBM_WAVE; k++) x ^= (uint32_t)s.red[k];
+            crc_out[b] = x;
+            if (err == CBM_BLOCK_OK && x != expect[b]) err = CBM_BLOCK_CRC;
+            status[b] = err | (err != CBM_BLOCK_OK ? site << 8 : 0);
+            PHASE(STAT_STORE);
+            if (stats)
+                for (int k = 0; k < STAT_N; k++) atomicAdd(&stats[k], (unsigned long long)acc[k]), acc[k] = 0;
+            if (err != CBM_BLOCK_OK) atomicAdd((unsigned long long*)bad_count, 1ull);
+        }
+        __syncthreads();
+    }
+#undef PHASE
+}
+
+}  // namespace cbm

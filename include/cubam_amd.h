@@ -74,7 +74,8 @@ int cbm_bgzf_scan(const uint8_t* buf_d, int64_t n, int eof, int64_t cap, int64_t
 
 /* ------------------------------------------------------------------ inflate + CRC32 (kernel 2) */
 
-/* Per-block status written by cbm_inflate. */
+/* Per-block status written by cbm_inflate (low byte; bits 8-15 of a failed block name the
+   check that failed, for diagnostics). */
 enum {
     CBM_BLOCK_OK = 0,
     CBM_BLOCK_DEFLATE = 1, /* invalid DEFLATE stream (bad block type, code, distance, overrun) */
@@ -82,12 +83,21 @@ enum {
     CBM_BLOCK_CRC = 3      /* CRC32 of the output differs from the trailer */
 };
 
+/* Inflate kernel variants (cbm_inflate_tune.variant). */
+enum {
+    CBM_INFLATE_WG256 = 0, /* default: a 256-thread workgroup decodes each block, 128-bit segments */
+    CBM_INFLATE_LANES = 1, /* each of blocks_per_wave lanes of a wavefront inflates its own block */
+    CBM_INFLATE_WG128 = 2, /* 128-thread workgroup per block, 256-bit segments */
+    CBM_INFLATE_WG512 = 3, /* 512-thread workgroup per block, 64-bit segments */
+    CBM_INFLATE_WG64 = 4   /* one wavefront per block, 512-bit segments */
+};
+
 /* Launch shape of the inflate kernel; zeros select the defaults (or the autotuned choice). */
 typedef struct {
-    int32_t blocks_per_wave; /* lanes of a wavefront that each inflate a block: 1..64 */
-    int32_t waves_per_group; /* wavefronts per workgroup: 1..16 */
-    int32_t layout;          /* LDS decode-table layout: 0 lane-major, 1 lane-interleaved */
-    int32_t groups;          /* workgroups launched (persistent); 0 = enough for all blocks */
+    int32_t variant;         /* CBM_INFLATE_* */
+    int32_t blocks_per_wave; /* CBM_INFLATE_LANES: lanes of a wavefront that each inflate a block, 1..64 */
+    int32_t waves_per_group; /* CBM_INFLATE_LANES: wavefronts per workgroup, 1..16 */
+    int32_t groups;          /* workgroups launched (persistent); 0 = as many as the CUs hold */
 } cbm_inflate_tune;
 
 /* Bytes of workspace cbm_inflate needs for n_blocks blocks. */

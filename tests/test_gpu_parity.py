@@ -69,13 +69,15 @@ def test_hetero_blocks_bytes_and_crc(cbm, oracle):
     assert (ost == 0).all() and ocrc.astype(np.int64).tolist() == expect
 
 
-@pytest.mark.parametrize("bpw,wpg", [(1, 1), (2, 4), (4, 2), (8, 4), (16, 3), (32, 1)])
-def test_inflate_every_launch_shape(cbm, bpw, wpg):
+@pytest.mark.parametrize("variant,bpw,wpg,groups", [(0, 0, 0, 0), (0, 0, 0, 7), (2, 0, 0, 0), (3, 0, 0, 0),
+                                                    (4, 0, 0, 0), (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0),
+                                                    (1, 32, 1, 0)])
+def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     from cubam_amd import _lib
 
     hb = _hetero(700, 3)
-    _say("shape", bpw, wpg)
-    tune = _lib.InflateTune(bpw, wpg, 0, 0)
+    _say("shape", variant, bpw, wpg, groups)
+    tune = _lib.InflateTune(variant, bpw, wpg, groups)
     blocks, inf = _gpu_blocks(hb.bgzf, tune)
     assert int(inf.result.item()) == 0
     assert inf.data[: inf.n].cpu().numpy().tobytes() == hb.raw.tobytes()
