@@ -231,25 +231,22 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
     const int64_t groups = tune ? tune->groups : 0;
     if (variant != CBM_INFLATE_LANES) {
         ARG(ws && ws_bytes >= cbm_inflate_workspace(n_blocks), "cbm_inflate: workspace too small");
-        int64_t* counter = (int64_t*)ws;  // [0] next block, [8..15] decoder statistics
-        if ((rc = fill(counter, 16, 0, st))) return rc;
+        int64_t* counter = (int64_t*)ws;  // [0] next block, [8..] decoder statistics
+        if ((rc = fill(counter, 8 + STAT_N, 0, st))) return rc;
         switch (variant) {
-            case CBM_INFLATE_WG256: return launch_wave<256, 128>(src, blk_off, bsize, isize, crc_expect, out_off,
-                                                                 n_blocks, dst, crc, status, result, groups, counter, st);
-            case CBM_INFLATE_WG128: return launch_wave<128, 256>(src, blk_off, bsize, isize, crc_expect, out_off,
-                                                                 n_blocks, dst, crc, status, result, groups, counter, st);
-            case CBM_INFLATE_WG512: return launch_wave<512, 64>(src, blk_off, bsize, isize, crc_expect, out_off,
-                                                                n_blocks, dst, crc, status, result, groups, counter, st);
-            case CBM_INFLATE_WG64: return launch_wave<64, 512>(src, blk_off, bsize, isize, crc_expect, out_off,
-                                                               n_blocks, dst, crc, status, result, groups, counter, st);
-            case 5: return launch_wave<256, 256>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
-                                                 status, result, groups, counter, st);
-            case 6: return launch_wave<128, 128>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
-                                                 status, result, groups, counter, st);
-            case 7: return launch_wave<64, 128>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
-                                                status, result, groups, counter, st);
-            case 8: return launch_wave<64, 256>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
-                                                status, result, groups, counter, st);
+#define WAVE_CASE(V, NT_, SEG_)                                                                               \
+    case V:                                                                                                   \
+        return launch_wave<NT_, SEG_>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, \
+                                      result, groups, counter, st);
+            WAVE_CASE(CBM_INFLATE_WG256, 256, 256)
+            WAVE_CASE(CBM_INFLATE_WG128, 128, 256)
+            WAVE_CASE(CBM_INFLATE_WG512, 512, 64)
+            WAVE_CASE(CBM_INFLATE_WG64, 64, 512)
+            WAVE_CASE(CBM_INFLATE_WG256_S128, 256, 128)
+            WAVE_CASE(CBM_INFLATE_WG128_S128, 128, 128)
+            WAVE_CASE(CBM_INFLATE_WG64_S128, 64, 128)
+            WAVE_CASE(CBM_INFLATE_WG64_S256, 64, 256)
+#undef WAVE_CASE
             default: ARG(false, "cbm_inflate: unknown variant");
         }
     }

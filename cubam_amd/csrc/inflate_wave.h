@@ -24,6 +24,13 @@
 
 namespace cbm {
 
+enum { SV_POS = 0, SV_FINAL, SV_TYPE, SV_ERR, SV_LEN, SV_SRC, SV_OP, SV_CLEAR, SV_TEND, SV_CHANGED, SV_DONE, SV_OK,
+       SV_BLOCK, SV_MAX, SV_SITE, SV_F0, SV_F1, SV_F2, SV_N };
+
+// matches each thread holds in pass D; copies move MD_PIECE bytes per round trip
+constexpr int MD_K = 8;
+constexpr int MD_PIECE = 16;
+
 template <int NT, int SEG>
 struct WaveSmem {
     static constexpr int IN_WORDS = NT * SEG / 32 + 8;  // staged compressed bits of a round
@@ -40,11 +47,9 @@ struct WaveSmem {
     int32_t lcount[16];
     int32_t lnext[16];
     int32_t chunkcnt[5][16];
-    int32_t sv[16];  // shared scalars (SV_*)
+    int32_t sv[SV_N];  // shared scalars (SV_*)
 };
 
-enum { SV_POS = 0, SV_FINAL, SV_TYPE, SV_ERR, SV_LEN, SV_SRC, SV_OP, SV_CLEAR, SV_TEND, SV_CHANGED, SV_DONE, SV_OK,
-       SV_BLOCK, SV_MAX, SV_SITE };
 
 // the check that failed, reported in bits 8-15 of a block's status
 enum { SITE_HEADER = 1, SITE_STORED = 2, SITE_TYPE = 3, SITE_CLCODE = 4, SITE_LENGTHS = 5, SITE_TABLES = 6,
@@ -54,7 +59,7 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 
 // decoder statistics (cycles of thread 0 per phase, and rounds), summed over blocks
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
-       STAT_N };
+       STAT_DSTEPS, STAT_LINKIT, STAT_N };
 
 // matches of a round, per workgroup, in global scratch: a block of at most 65,536 bytes has at
 // most 21,846 matches of 3 bytes or more
@@ -102,10 +107,11 @@ struct Sym {
     int dist;
 };
 
+// decode one symbol (a literal/length code and, for a length, its distance) from the next
+// 64 bits of the stream; a symbol uses at most 15 + 5 + 15 + 13 = 48 bits
 template <class S>
-__device__ __forceinline__ Sym decode_at(const S& s, int p, int wbase) {
+__device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
     Sym r;
-    uint64_t v = peek64(s, p, wbase);
     uint32_t e = s.lit[v & ((1u << LBITS) - 1)];
     int used;
     if (e & 0x8000) {
@@ -169,6 +175,32 @@ __device__ __forceinline__ Sym decode_at(const S& s, int p, int wbase) {
     r.dist = db + (int)(v & ((1u << de) - 1));
     r.bits = used + de;
     return r;
+}
+
+// A window on the staged stream: decodes symbol after symbol, re-reading LDS only when fewer
+// than 48 bits are left (after a literal of at most 15 bits, the next symbol still fits)
+struct Win {
+    int p;      // stream position of v's bit 0
+    int avail;  // valid bits in v
+    uint64_t v;
+};
+template <class S>
+__device__ __forceinline__ void win_at(Win& w, const S& s, int p, int wb) {
+    w.p = p;
+    w.v = peek64(s, p, wb);
+    w.avail = 64;
+}
+template <class S>
+__device__ __forceinline__ Sym win_next(Win& w, const S& s, int wb) {
+    if (w.avail < 48) {
+        w.v = peek64(s, w.p, wb);
+        w.avail = 64;
+    }
+    Sym y = decode_v(s, w.v);
+    w.v >>= y.bits;
+    w.avail -= y.bits;
+    w.p += y.bits;
+    return y;
 }
 
 // ------------------------------------------------------------------ cooperative table build
@@ -311,7 +343,7 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
     const int tid = threadIdx.x;
     uint64_t* mlist = mlists + (size_t)blockIdx.x * MLIST_CAP;
     // cycles per phase, kept by thread 0 and added to stats[] once per block
-    long long acc[STAT_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long acc[STAT_N] = {};
     long long tlast = tid == 0 ? clock64() : 0;
 #define PHASE(k)                          \
     do {                                  \
@@ -438,31 +470,40 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 if (tid == 0) {
                     int p = p1, n = 0, total = hlit + hdist, err = 0;
                     int wb = wb1;
+                    uint64_t v = 0;
+                    int avail = 0;
                     while (n < total) {
-                        if ((p >> 5) - wb + 3 >= S::IN_WORDS) break;  // needs a new staging (rare)
-                        uint64_t v = peek64(s, p, wb);
+                        if (avail < 14) {  // a code-length symbol uses at most 7 + 7 bits
+                            if ((p >> 5) - wb + 3 >= S::IN_WORDS) break;  // needs a new staging (rare)
+                            v = peek64(s, p, wb);
+                            avail = 64;
+                        }
                         uint32_t e = s.dist[v & 127];
                         int l = e & 15, sym = (e >> 4) & 0x1ff;
                         if (!l) { err = 1; break; }
                         p += l;
                         v >>= l;
+                        avail -= l;
                         if (sym < 16) {
                             s.lens[n++] = (uint8_t)sym;
                         } else {
-                            int rep;
+                            int rep, xb;
                             uint8_t val = 0;
                             if (sym == 16) {
                                 if (n == 0) { err = 1; break; }
                                 val = s.lens[n - 1];
                                 rep = 3 + (int)(v & 3);
-                                p += 2;
+                                xb = 2;
                             } else if (sym == 17) {
                                 rep = 3 + (int)(v & 7);
-                                p += 3;
+                                xb = 3;
                             } else {
                                 rep = 11 + (int)(v & 127);
-                                p += 7;
+                                xb = 7;
                             }
+                            p += xb;
+                            v >>= xb;
+                            avail -= xb;
                             if (n + rep > total) { err = 1; break; }
                             while (rep--) s.lens[n++] = val;
                         }
@@ -514,16 +555,18 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 for (int k = 0; k < SEG / 32; k++) seen[k] = 0;
                 int p = a, stop = STOP_NONE;
                 if (active) {
-                    while (p < aend) {
-                        int r = p - a;
+                    Win w;
+                    win_at(w, s, a, wb);
+                    while (w.p < aend) {
+                        int r = w.p - a;
 #pragma unroll
                         for (int k = 0; k < SEG / 32; k++)
                             if ((r >> 5) == k) seen[k] |= 1u << (r & 31);
-                        Sym y = decode_at(s, p, wb);
-                        p += y.bits;
+                        Sym y = win_next(w, s, wb);
                         if (y.kind == 2) { stop = STOP_EOB; break; }
-                        if (y.kind == 3 || p > in_bits) { stop = STOP_BAD; break; }
+                        if (y.kind == 3 || w.p > in_bits) { stop = STOP_BAD; break; }
                     }
+                    p = w.p;
                 }
                 s.exit_[tid] = p;
                 s.stop_[tid] = active ? stop : STOP_BAD;
@@ -533,19 +576,23 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 __syncthreads();
                 PHASE(STAT_PASS_A);
                 // link the walks
-                for (;;) {
+                if (tid == 0) s.sv[SV_F0] = s.sv[SV_F1] = s.sv[SV_F2] = 0;
+                __syncthreads();
+                for (int it = 0;; it++) {
+                    // flag it % 3 gathers this iteration's changes; it + 1 was cleared last time
                     int want = tid == 0 ? base : s.exit_[tid - 1];
                     int pstop = tid == 0 ? STOP_NONE : s.stop_[tid - 1];
-                    if (tid == 0 && threadIdx.x == 0) s.sv[SV_CHANGED] = 0;
+                    if (tid == 0) s.sv[SV_F0 + (it + 1) % 3] = 0;
                     __syncthreads();
                     bool redo = active && pstop == STOP_NONE && want != entry;
                     int np = p, nstop = stop;
                     if (redo) {
-                        np = want;
                         nstop = STOP_NONE;
                         bool synced = false;
-                        while (np < aend) {
-                            int r = np - a;
+                        Win w;
+                        win_at(w, s, want, wb);
+                        while (w.p < aend) {
+                            int r = w.p - a;
                             if (r >= 0) {
                                 uint32_t bit = 0;
 #pragma unroll
@@ -553,25 +600,25 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                                     if ((r >> 5) == k) bit = (seen[k] >> (r & 31)) & 1;
                                 if (bit) { synced = true; break; }
                             }
-                            Sym y = decode_at(s, np, wb);
-                            np += y.bits;
+                            Sym y = win_next(w, s, wb);
                             if (y.kind == 2) { nstop = STOP_EOB; break; }
-                            if (y.kind == 3 || np > in_bits) { nstop = STOP_BAD; break; }
+                            if (y.kind == 3 || w.p > in_bits) { nstop = STOP_BAD; break; }
                         }
+                        np = w.p;
                         if (synced) {
                             np = a_exit;
                             nstop = a_stop;
                         }
                         entry = want;
-                        if (np != p || nstop != stop) s.sv[SV_CHANGED] = 1;
+                        if (np != p || nstop != stop) s.sv[SV_F0 + it % 3] = 1;
                         p = np;
                         stop = nstop;
                         s.exit_[tid] = p;
                         s.stop_[tid] = stop;
                     }
                     __syncthreads();
-                    int changed = s.sv[SV_CHANGED];
-                    __syncthreads();
+                    const int changed = s.sv[SV_F0 + it % 3];
+                    if (tid == 0) acc[STAT_LINKIT]++;
                     if (!changed) break;
                 }
                 PHASE(STAT_LINK);
@@ -585,11 +632,11 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 // pass B: output bytes and matches of each live walk, entry .. exit
                 int cnt = 0, nmatch = 0;
                 if (live) {
-                    int q = tid == 0 ? base : s.exit_[tid - 1];
                     const int e = s.exit_[tid];
-                    while (q < e) {
-                        Sym y = decode_at(s, q, wb);
-                        q += y.bits;
+                    Win w;
+                    win_at(w, s, tid == 0 ? base : s.exit_[tid - 1], wb);
+                    while (w.p < e) {
+                        Sym y = win_next(w, s, wb);
                         if (y.kind == 0) cnt += 1;
                         else if (y.kind == 1) cnt += y.val, nmatch++;
                         else break;
@@ -614,19 +661,19 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 PHASE(STAT_PASS_B);
                 // pass C: literals go to their place; matches go to the list, in output order
                 if (live && cnt) {
-                    int q = tid == 0 ? base : s.exit_[tid - 1];
                     const int e = s.exit_[tid];
-                    int w = op + ob, k = mbase;
+                    int o = op + ob, k = mbase;
                     bool bad = false;
-                    while (q < e) {
-                        Sym y = decode_at(s, q, wb);
-                        q += y.bits;
+                    Win w;
+                    win_at(w, s, tid == 0 ? base : s.exit_[tid - 1], wb);
+                    while (w.p < e) {
+                        Sym y = win_next(w, s, wb);
                         if (y.kind == 0) {
-                            s.out[w++] = (uint8_t)y.val;
+                            s.out[o++] = (uint8_t)y.val;
                         } else if (y.kind == 1) {
-                            if (y.dist > w) { bad = true; break; }
-                            mlist[k++] = (uint64_t)w | ((uint64_t)y.dist << 17) | ((uint64_t)y.val << 33);
-                            w += y.val;
+                            if (y.dist > o) { bad = true; break; }
+                            mlist[k++] = (uint64_t)o | ((uint64_t)y.dist << 17) | ((uint64_t)y.val << 33);
+                            o += y.val;
                         } else {
                             break;
                         }
@@ -640,12 +687,77 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 __syncthreads();
                 if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
                 PHASE(STAT_PASS_C);
-                // pass D: one wavefront applies the matches in order, 64 lanes per copy; every
-                // source byte precedes its match, so it is final when the match is reached
-                if (tid < CBM_WAVE) {
+                // pass D: apply the round's matches. Let F be the destination of the first match
+                // not yet applied: every byte before F is final, so every match whose source ends
+                // at or before F can be applied now, wherever it is in the list. Each step applies
+                // all of those at once (each thread holds up to MD_K matches of the list, in
+                // order) and F moves on to the first one left. Rounds with more matches than the
+                // threads hold go through one wavefront in list order.
+                if (mtotal <= MD_K * NT) {
+                    uint64_t mm[MD_K];
+                    uint32_t left = 0;
+#pragma unroll
+                    for (int u = 0; u < MD_K; u++) {
+                        const int k = tid + u * NT;
+                        mm[u] = k < mtotal ? __atomic_load_n(&mlist[k], __ATOMIC_RELAXED) : 0;
+                        if (k < mtotal) left |= 1u << u;
+                    }
+                    if (tid == 0) {
+                        s.sv[SV_F0] = mtotal ? (int)(mm[0] & 0x1ffff) : 0x7fffffff;
+                        s.sv[SV_F0 + 1] = 0x7fffffff;
+                        s.sv[SV_F0 + 2] = 0x7fffffff;
+                    }
+                    __syncthreads();
+                    for (int step = 0;; step++) {
+                        const int F = s.sv[SV_F0 + step % 3];
+                        if (F == 0x7fffffff) break;
+#pragma unroll
+                        for (int u = 0; u < MD_K; u++) {
+                            if (!(left & (1u << u))) continue;
+                            const uint64_t m = mm[u];
+                            const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+                            const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
+                            if (src + (dist < len ? dist : len) > F) continue;
+                            left &= ~(1u << u);
+                            if (dist >= len) {
+                                for (int lo = 0; lo < len; lo += MD_PIECE) {
+                                    uint8_t t16[MD_PIECE];
+#pragma unroll
+                                    for (int q = 0; q < MD_PIECE; q++) t16[q] = lo + q < len ? s.out[src + lo + q] : 0;
+#pragma unroll
+                                    for (int q = 0; q < MD_PIECE; q++)
+                                        if (lo + q < len) s.out[dest + lo + q] = t16[q];
+                                }
+                            } else if (dist >= 8) {
+                                for (int q = 0; q < len; q++) s.out[dest + q] = s.out[src + q];
+                            } else {
+                                // the dist bytes before dest repeat: read them once, then only write
+                                uint64_t pat = 0;
+                                for (int q = 0; q < dist; q++) pat |= (uint64_t)s.out[src + q] << (8 * q);
+                                int r = 0;
+                                for (int q = 0; q < len; q++) {
+                                    s.out[dest + q] = (uint8_t)(pat >> (8 * r));
+                                    r = r + 1 == dist ? 0 : r + 1;
+                                }
+                            }
+                        }
+                        // the next frontier: a minimum within the wavefront, then one atomic per wavefront
+                        int mine = left ? (int)(mm[__ffs(left) - 1] & 0x1ffff) : 0x7fffffff;
+#pragma unroll
+                        for (int d = 32; d; d >>= 1) {
+                            int o = __shfl_xor(mine, d, CBM_WAVE);
+                            mine = o < mine ? o : mine;
+                        }
+                        if ((tid % CBM_WAVE) == 0 && mine != 0x7fffffff) atomicMin(&s.sv[SV_F0 + (step + 1) % 3], mine);
+                        if (tid == 0) s.sv[SV_F0 + (step + 2) % 3] = 0x7fffffff;
+                        __syncthreads();
+                        if (tid == 0) acc[STAT_DSTEPS]++;
+                    }
+                } else if (tid < CBM_WAVE) {
                     const int lane = tid;
                     for (int k0 = 0; k0 < mtotal; k0 += CBM_WAVE) {
-                        uint64_t mine = k0 + lane < mtotal ? __atomic_load_n(&mlist[k0 + lane], __ATOMIC_RELAXED) : 0;
+                        const uint64_t mine =
+                            k0 + lane < mtotal ? __atomic_load_n(&mlist[k0 + lane], __ATOMIC_RELAXED) : 0;
                         const int nb = mtotal - k0 < CBM_WAVE ? mtotal - k0 : CBM_WAVE;
                         for (int j = 0; j < nb; j++) {
                             const uint64_t m = __shfl(mine, j, CBM_WAVE);

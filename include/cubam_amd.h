@@ -83,13 +83,19 @@ enum {
     CBM_BLOCK_CRC = 3      /* CRC32 of the output differs from the trailer */
 };
 
-/* Inflate kernel variants (cbm_inflate_tune.variant). */
+/* Inflate kernel variants (cbm_inflate_tune.variant): a workgroup of NT threads decodes each
+   block, each thread SEG bits of a round; or one block per lane (CBM_INFLATE_LANES). */
 enum {
-    CBM_INFLATE_WG256 = 0, /* default: a 256-thread workgroup decodes each block, 128-bit segments */
-    CBM_INFLATE_LANES = 1, /* each of blocks_per_wave lanes of a wavefront inflates its own block */
-    CBM_INFLATE_WG128 = 2, /* 128-thread workgroup per block, 256-bit segments */
-    CBM_INFLATE_WG512 = 3, /* 512-thread workgroup per block, 64-bit segments */
-    CBM_INFLATE_WG64 = 4   /* one wavefront per block, 512-bit segments */
+    CBM_INFLATE_WG256 = 0,      /* default: NT 256, SEG 256 */
+    CBM_INFLATE_LANES = 1,      /* blocks_per_wave lanes of a wavefront each inflate a block */
+    CBM_INFLATE_WG128 = 2,      /* NT 128, SEG 256 */
+    CBM_INFLATE_WG512 = 3,      /* NT 512, SEG 64 */
+    CBM_INFLATE_WG64 = 4,       /* NT 64, SEG 512 */
+    CBM_INFLATE_WG256_S128 = 5, /* NT 256, SEG 128 */
+    CBM_INFLATE_WG128_S128 = 6, /* NT 128, SEG 128 */
+    CBM_INFLATE_WG64_S128 = 7,  /* NT 64, SEG 128 */
+    CBM_INFLATE_WG64_S256 = 8,  /* NT 64, SEG 256 */
+    CBM_INFLATE_VARIANTS = 9
 };
 
 /* Launch shape of the inflate kernel; zeros select the defaults (or the autotuned choice). */
