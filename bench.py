@@ -180,7 +180,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "cbm_inflate (inflate_lanes + crc_blocks)",
+            "kernel": _kernel_name(),
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -194,6 +194,16 @@ def main():
         res["cpu_baseline"] = {"value": round(v, 3), "unit": "GB/s", "cores": cores, "kind": "port",
                                "sample": sample}
     print(json.dumps(res), flush=True)
+
+
+def _kernel_name() -> str:
+    from cubam_amd import _decode
+    from cubam_amd._lib import INFLATE_SHAPES
+
+    shape = INFLATE_SHAPES[_decode.get_inflate_tune()[0]]
+    if shape is None:
+        return "cbm_inflate: inflate_lanes + crc_blocks"
+    return f"cbm_inflate: inflate_wave<{shape[0]}, {shape[1]}> (CRC32 fused)"
 
 
 def _decode_header(raw: bytes):
