@@ -77,6 +77,10 @@ SYMBOLS = (
     "cbm_record_chain",
     "cbm_record_chain_repair",
     "cbm_record_offsets",
+    "cbm_record_spans_workspace",
+    "cbm_record_chain_spans",
+    "cbm_record_chain_spans_repair",
+    "cbm_record_offsets_spans",
     "cbm_scatter_workspace",
     "cbm_scatter",
     "cbm_last_error",
@@ -116,6 +120,13 @@ def load():
     L.cbm_record_chain_repair.argtypes = [vp, i64, i64, i32, vp, vp, i64, vp]
     L.cbm_record_offsets.restype = ctypes.c_int
     L.cbm_record_offsets.argtypes = [vp, i64, i64, vp, i64, vp, i64, vp]
+    L.cbm_record_spans_workspace.restype = i64
+    L.cbm_record_spans_workspace.argtypes = [i64]
+    for f in (L.cbm_record_chain_spans, L.cbm_record_chain_spans_repair):
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp]
+    L.cbm_record_offsets_spans.restype = ctypes.c_int
+    L.cbm_record_offsets_spans.argtypes = [vp, i64, vp, i64, vp, i64, vp]
     L.cbm_scatter_workspace.restype = i64
     L.cbm_scatter_workspace.argtypes = [i64]
     L.cbm_scatter.restype = ctypes.c_int

@@ -105,6 +105,63 @@ __global__ void rec_pred_fill(RecWs w) {
         w.pred[k] = w.f[k] >= 0 ? 1 : 0;
 }
 
+// Host side of the chain repair: walk each span's chain from its start, re-walking every
+// segment whose speculative entry is not the true exit of the walk before it.
+static int repair_load(const RecWs& w, std::vector<int64_t>& f, std::vector<int64_t>& e, std::vector<int64_t>& flag,
+                       std::vector<char>& keep, hipStream_t st) {
+    int64_t ns = w.nseg;
+    f.assign(ns, 0);
+    e.assign(ns, 0);
+    flag.assign(ns, 0);
+    keep.assign(ns, 0);
+    if (hipMemcpyAsync(f.data(), w.f, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(e.data(), w.e, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(flag.data(), w.flag, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("record chain repair: copy failed");
+        return CBM_ERR_LAUNCH;
+    }
+    return CBM_OK;
+}
+
+static int repair_span(const uint8_t* data, int64_t s0, int64_t end, int64_t k0, int32_t n_ref, const RecWs& w,
+                       std::vector<int64_t>& f, std::vector<int64_t>& e, std::vector<int64_t>& flag,
+                       std::vector<char>& keep, hipStream_t st) {
+    int64_t p = s0;
+    while (p < end) {
+        int64_t k = k0 + (p - s0) / REC_SEG;
+        if (f[k] != p) {
+            // rec_walk_one places segment k at start + k * REC_SEG
+            hipLaunchKernelGGL(rec_walk_one, dim3(1), dim3(1), 0, st, data, s0 - k0 * REC_SEG, end, n_ref, w, k, p);
+            int rc = check_launch("rec_walk_one");
+            if (rc) return rc;
+            if (hipMemcpyAsync(&e[k], w.e + k, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(&flag[k], w.flag + k, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                set_error("record chain repair: copy failed");
+                return CBM_ERR_LAUNCH;
+            }
+            f[k] = p;
+        }
+        keep[k] = 1;
+        if (flag[k]) break;
+        p = e[k];
+    }
+    return CBM_OK;
+}
+
+// walks off the chains no longer count
+static int repair_store(const RecWs& w, std::vector<int64_t>& f, const std::vector<char>& keep, hipStream_t st) {
+    for (int64_t k = 0; k < w.nseg; k++)
+        if (!keep[k] && f[k] >= 0) f[k] = -1;
+    if (hipMemcpyAsync(w.f, f.data(), w.nseg * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("record chain repair: copy failed");
+        return CBM_ERR_LAUNCH;
+    }
+    return CBM_OK;
+}
+
 // workgroups of the persistent inflate launch: the LDS of a CU holds two block buffers
 static int64_t inflate_max_groups() { return (int64_t)device_cus() * 2; }
 
@@ -323,51 +380,18 @@ int cbm_record_chain_repair(const uint8_t* data, int64_t start, int64_t n, int32
     RecWs w = rec_ws_layout(ws, n - start);
     int64_t ns = w.nseg;
     if (ns == 0) return CBM_OK;
-    std::vector<int64_t> f(ns), e(ns), flag(ns);
-    if (hipMemcpyAsync(f.data(), w.f, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(e.data(), w.e, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(flag.data(), w.flag, ns * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-        set_error("cbm_record_chain_repair: copy failed");
-        return CBM_ERR_LAUNCH;
-    }
-    std::vector<char> keep(ns, 0);
-    int64_t p = start;
-    while (p < n) {
-        int64_t k = (p - start) / REC_SEG;
-        if (f[k] != p) {
-            hipLaunchKernelGGL(rec_walk_one, dim3(1), dim3(1), 0, st, data, start, n, n_ref, w, k, p);
-            int rc = check_launch("rec_walk_one");
-            if (rc) return rc;
-            if (hipMemcpyAsync(&e[k], w.e + k, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipMemcpyAsync(&flag[k], w.flag + k, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess) {
-                set_error("cbm_record_chain_repair: copy failed");
-                return CBM_ERR_LAUNCH;
-            }
-            f[k] = p;
-        }
-        keep[k] = 1;
-        if (flag[k]) break;
-        p = e[k];
-    }
-    // walks off the chain no longer count
-    std::vector<int64_t> drop;
-    for (int64_t k = 0; k < ns; k++)
-        if (!keep[k] && f[k] >= 0) drop.push_back(k);
-    for (int64_t k : drop) f[k] = -1;
-    if (hipMemcpyAsync(w.f, f.data(), ns * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-        set_error("cbm_record_chain_repair: copy failed");
-        return CBM_ERR_LAUNCH;
-    }
+    std::vector<int64_t> f, e, flag;
+    std::vector<char> keep;
+    int rc = repair_load(w, f, e, flag, keep, st);
+    if (rc) return rc;
+    if ((rc = repair_span(data, start, n, 0, n_ref, w, f, e, flag, keep, st))) return rc;
+    if ((rc = repair_store(w, f, keep, st))) return rc;
     // counts of dropped walks are masked by f < 0 in rec_sum and rec_emit; recount
     hipLaunchKernelGGL(rec_result_init, dim3(1), dim3(64), 0, st, result, start, w);
     hipLaunchKernelGGL(rec_pred_fill, dim3(256), dim3(256), 0, st, w);
     unsigned g = (unsigned)((ns + REC_NT - 1) / REC_NT);
     hipLaunchKernelGGL(rec_sum, dim3(g), dim3(REC_NT), 0, st, w, result);
-    int rc = check_launch("rec_sum");
-    if (rc) return rc;
+    if ((rc = check_launch("rec_sum"))) return rc;
     // cnt of dropped segments must not enter the scan
     hipLaunchKernelGGL(rec_zero_dropped, dim3(g), dim3(REC_NT), 0, st, w);
     if ((rc = check_launch("rec_zero_dropped"))) return rc;
@@ -388,6 +412,98 @@ int cbm_record_offsets(const uint8_t* data, int64_t start, int64_t n, const void
     RecWs w = rec_ws_layout((void*)ws, n - start);
     unsigned g = (unsigned)((w.nseg + REC_NT - 1) / REC_NT);
     hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, start, n, w, offsets, n_rec);
+    return check_launch("rec_emit");
+}
+
+// ------------------------------------------------------------------ kernel 3 over many spans
+int64_t cbm_record_spans_workspace(int64_t n_seg) { return n_seg < 0 ? -1 : rec_ws_bytes_seg(n_seg); }
+
+static int spans_finish(SpanTab t, RecWs w, int64_t* span_rec, hipStream_t st) {
+    int rc = exclusive_scan(LoadI64{w.cnt}, StoreI64{w.base}, (const int64_t*)nullptr, w.nseg, w.scan_ws,
+                            (int64_t*)nullptr, st);
+    if (rc) return rc;
+    unsigned g = (unsigned)((t.n + 1 + 255) / 256);
+    hipLaunchKernelGGL(recs_bases, dim3(g), dim3(256), 0, st, t, w, span_rec);
+    return check_launch("recs_bases");
+}
+
+int cbm_record_chain_spans(const uint8_t* data, const int64_t* span_start, const int64_t* span_end,
+                           const int64_t* seg_base, const int32_t* span_nref, int64_t n_spans, int64_t n_seg,
+                           int64_t* result, int64_t* span_rec, void* ws, int64_t ws_bytes, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(n_spans >= 1 && n_seg >= 0 && result && span_rec && ws, "cbm_record_chain_spans: bad arguments");
+    ARG(span_start && span_end && seg_base && span_nref, "cbm_record_chain_spans: null span table");
+    ARG(ws_bytes >= rec_ws_bytes_seg(n_seg), "cbm_record_chain_spans: workspace too small");
+    RecWs w = rec_ws_layout_seg(ws, n_seg);
+    SpanTab t{span_start, span_end, seg_base, span_nref, n_spans};
+    hipLaunchKernelGGL(rec_result_init, dim3(1), dim3(64), 0, st, result, (int64_t)0, w);
+    int rc = check_launch("rec_result_init");
+    if (rc) return rc;
+    if (n_seg > 0) {
+        ARG(data, "cbm_record_chain_spans: null data");
+        unsigned g = (unsigned)((n_seg + REC_NT - 1) / REC_NT);
+        hipLaunchKernelGGL(recs_walk, dim3(g), dim3(REC_NT), 0, st, data, t, w);
+        if ((rc = check_launch("recs_walk"))) return rc;
+        hipLaunchKernelGGL(recs_links, dim3(g), dim3(REC_NT), 0, st, t, w);
+        if ((rc = check_launch("recs_links"))) return rc;
+        hipLaunchKernelGGL(recs_sum, dim3(g), dim3(REC_NT), 0, st, t, w, result);
+        if ((rc = check_launch("recs_sum"))) return rc;
+        hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
+        if ((rc = check_launch("rec_result_links"))) return rc;
+    }
+    return spans_finish(t, w, span_rec, st);
+}
+
+int cbm_record_chain_spans_repair(const uint8_t* data, const int64_t* span_start, const int64_t* span_end,
+                                  const int64_t* seg_base, const int32_t* span_nref, int64_t n_spans, int64_t n_seg,
+                                  int64_t* result, int64_t* span_rec, void* ws, int64_t ws_bytes, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(n_spans >= 1 && n_seg >= 0 && result && span_rec && ws, "cbm_record_chain_spans_repair: bad arguments");
+    ARG(span_start && span_end && seg_base && span_nref, "cbm_record_chain_spans_repair: null span table");
+    ARG(ws_bytes >= rec_ws_bytes_seg(n_seg), "cbm_record_chain_spans_repair: workspace too small");
+    if (n_seg == 0) return CBM_OK;
+    RecWs w = rec_ws_layout_seg(ws, n_seg);
+    SpanTab t{span_start, span_end, seg_base, span_nref, n_spans};
+    std::vector<int64_t> s0(n_spans), s1(n_spans), kb(n_spans + 1);
+    std::vector<int32_t> nr(n_spans);
+    if (hipMemcpyAsync(s0.data(), span_start, n_spans * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(s1.data(), span_end, n_spans * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(kb.data(), seg_base, (n_spans + 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(nr.data(), span_nref, n_spans * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("cbm_record_chain_spans_repair: copy failed");
+        return CBM_ERR_LAUNCH;
+    }
+    std::vector<int64_t> f, e, flag;
+    std::vector<char> keep;
+    int rc = repair_load(w, f, e, flag, keep, st);
+    if (rc) return rc;
+    for (int64_t s = 0; s < n_spans; s++)
+        if ((rc = repair_span(data, s0[s], s1[s], kb[s], nr[s], w, f, e, flag, keep, st))) return rc;
+    if ((rc = repair_store(w, f, keep, st))) return rc;
+    hipLaunchKernelGGL(rec_result_init, dim3(1), dim3(64), 0, st, result, (int64_t)0, w);
+    hipLaunchKernelGGL(rec_pred_fill, dim3(256), dim3(256), 0, st, w);
+    unsigned g = (unsigned)((n_seg + REC_NT - 1) / REC_NT);
+    hipLaunchKernelGGL(recs_sum, dim3(g), dim3(REC_NT), 0, st, t, w, result);
+    if ((rc = check_launch("recs_sum"))) return rc;
+    hipLaunchKernelGGL(rec_zero_dropped, dim3(g), dim3(REC_NT), 0, st, w);
+    if ((rc = check_launch("rec_zero_dropped"))) return rc;
+    hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
+    if ((rc = check_launch("rec_result_links"))) return rc;
+    if ((rc = spans_finish(t, w, span_rec, st))) return rc;
+    return hipStreamSynchronize(st) == hipSuccess ? CBM_OK : CBM_ERR_LAUNCH;
+}
+
+int cbm_record_offsets_spans(const uint8_t* data, int64_t n_seg, const void* ws, int64_t ws_bytes, int64_t* offsets,
+                             int64_t n_rec, void* stream) {
+    ARG(ws && n_seg >= 0 && n_rec >= 0, "cbm_record_offsets_spans: bad arguments");
+    ARG(ws_bytes >= rec_ws_bytes_seg(n_seg), "cbm_record_offsets_spans: workspace too small");
+    if (n_rec == 0 || n_seg == 0) return CBM_OK;
+    ARG(data && offsets, "cbm_record_offsets_spans: null pointer");
+    RecWs w = rec_ws_layout_seg((void*)ws, n_seg);
+    unsigned g = (unsigned)((n_seg + REC_NT - 1) / REC_NT);
+    hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, (int64_t)0, (int64_t)0, w,
+                       offsets, n_rec);
     return check_launch("rec_emit");
 }
 

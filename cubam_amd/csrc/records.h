@@ -38,9 +38,8 @@ struct RecWs {
 
 inline int64_t rec_nseg(int64_t len) { return (len + REC_SEG - 1) / REC_SEG; }
 
-inline RecWs rec_ws_layout(void* ws, int64_t len) {
+inline RecWs rec_ws_layout_seg(void* ws, int64_t nseg) {
     RecWs w;
-    int64_t nseg = rec_nseg(len);
     int64_t* p = (int64_t*)ws;
     w.nseg = nseg;
     w.hdr = p;
@@ -58,10 +57,11 @@ inline RecWs rec_ws_layout(void* ws, int64_t len) {
     return w;
 }
 
-inline int64_t rec_ws_bytes(int64_t len) {
-    int64_t nseg = rec_nseg(len);
-    return 8 * (8 + 9 * nseg + 1) + scan_workspace(nseg) + 64;
-}
+inline RecWs rec_ws_layout(void* ws, int64_t len) { return rec_ws_layout_seg(ws, rec_nseg(len)); }
+
+inline int64_t rec_ws_bytes_seg(int64_t nseg) { return 8 * (8 + 9 * nseg + 1) + scan_workspace(nseg) + 64; }
+
+inline int64_t rec_ws_bytes(int64_t len) { return rec_ws_bytes_seg(rec_nseg(len)); }
 
 // a record start confirmed by the record after it (or by the end of the data)
 __device__ __forceinline__ bool record_start(const uint8_t* b, int64_t n, int64_t o, int n_ref) {
@@ -192,6 +192,106 @@ __global__ void __launch_bounds__(REC_NT) rec_emit(const uint8_t* b, int64_t sta
         off[idx + i] = o;
         o += 4 + (int64_t)ld_u32(b, o);
     }
+}
+
+// ------------------------------------------------------------------ kernel 3 over many spans
+// A batch of requests inflates into one buffer; the records of request s are the bytes
+// [start[s], end[s]) of it, both record boundaries. Each span is cut into its own
+// segments (segbase[s] is the first), so one launch sequence chains every span. A span
+// must end exactly at its end: a record that runs past it, or a tail of fewer bytes than
+// a record, marks the span invalid.
+struct SpanTab {
+    const int64_t* start;
+    const int64_t* end;
+    const int64_t* segbase;  // n + 1 entries, nondecreasing
+    const int32_t* n_ref;    // references of the file each span comes from
+    int64_t n;
+};
+
+// the span holding segment k: the largest s with segbase[s] <= k (empty spans are skipped)
+__device__ __forceinline__ int64_t span_of(const SpanTab& t, int64_t k) {
+    int64_t lo = 0, hi = t.n - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (t.segbase[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(REC_NT) recs_walk(const uint8_t* b, SpanTab t, RecWs w) {
+    int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
+    if (k >= w.nseg) return;
+    int64_t s = span_of(t, k);
+    int64_t s0 = t.start[s], n = t.end[s], k0 = t.segbase[s];
+    int n_ref = t.n_ref[s];
+    int64_t lo = s0 + (k - k0) * REC_SEG, hi = lo + REC_SEG < n ? lo + REC_SEG : n;
+    int64_t f = -1;
+    if (k == k0) f = s0;
+    else
+        for (int64_t o = lo; o < hi; o++)
+            if (record_start(b, n, o, n_ref)) { f = o; break; }
+    w.f[k] = f;
+    w.pred[k] = 0;
+    WalkOut r = {-1, 0, 0, 0, 0, 0};
+    if (f >= 0) r = walk(b, n, f, hi, n_ref);
+    w.e[k] = r.e;
+    w.cnt[k] = r.cnt;
+    w.cig[k] = r.cig;
+    w.seqb[k] = r.seqb;
+    w.qual[k] = r.qual;
+    w.flag[k] = r.flag;
+}
+
+__global__ void __launch_bounds__(REC_NT) recs_links(SpanTab t, RecWs w) {
+    int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
+    int64_t bad = 0;
+    if (k < w.nseg && w.f[k] >= 0 && !w.flag[k]) {
+        int64_t s = span_of(t, k);
+        int64_t j = t.segbase[s] + (w.e[k] - t.start[s]) / REC_SEG;
+        if (j >= t.segbase[s + 1] || w.f[j] != w.e[k]) bad++;
+        else {
+            for (int64_t i = k + 1; i < j; i++)
+                if (w.f[i] >= 0) bad++;
+            atomicAdd((unsigned long long*)&w.pred[j], 1ull);
+        }
+    }
+    bad = wave_sum(bad);
+    if (lane_id() == 0 && bad) atomicAdd((unsigned long long*)&w.hdr[0], (unsigned long long)bad);
+}
+
+__global__ void __launch_bounds__(REC_NT) recs_sum(SpanTab t, RecWs w, int64_t* result) {
+    int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
+    int64_t bad = 0, cnt = 0, cig = 0, seqb = 0, qual = 0, inv = 0;
+    if (k < w.nseg && w.f[k] >= 0) {
+        int64_t s = span_of(t, k);
+        if (k > t.segbase[s] && w.pred[k] != 1) bad++;
+        cnt = w.cnt[k];
+        cig = w.cig[k];
+        seqb = w.seqb[k];
+        qual = w.qual[k];
+        if (w.flag[k] == SEG_END_INVALID || (w.flag[k] && w.e[k] != t.end[s])) inv++;
+    }
+    bad = wave_sum(bad);
+    cnt = wave_sum(cnt);
+    cig = wave_sum(cig);
+    seqb = wave_sum(seqb);
+    qual = wave_sum(qual);
+    inv = wave_sum(inv);
+    if (lane_id() == 0) {
+        if (bad) atomicAdd((unsigned long long*)&w.hdr[0], (unsigned long long)bad);
+        if (cnt) atomicAdd((unsigned long long*)&result[CBM_REC_N], (unsigned long long)cnt);
+        if (cig) atomicAdd((unsigned long long*)&result[CBM_REC_CIGAR], (unsigned long long)cig);
+        if (seqb) atomicAdd((unsigned long long*)&result[CBM_REC_SEQ], (unsigned long long)seqb);
+        if (qual) atomicAdd((unsigned long long*)&result[CBM_REC_QUAL], (unsigned long long)qual);
+        if (inv) atomicAdd((unsigned long long*)&result[CBM_REC_INVALID], (unsigned long long)inv);
+    }
+}
+
+// first record of each span (span_rec[s] = base of its first segment, span_rec[n] = total)
+__global__ void recs_bases(SpanTab t, RecWs w, int64_t* span_rec) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s <= t.n; s += (int64_t)gridDim.x * blockDim.x)
+        span_rec[s] = w.base[t.segbase[s]];
 }
 
 // ------------------------------------------------------------------ kernel 4: scatter

@@ -171,6 +171,32 @@ int cbm_record_chain_repair(const uint8_t* data_d, int64_t start, int64_t n, int
 int cbm_record_offsets(const uint8_t* data_d, int64_t start, int64_t n, const void* ws_d, int64_t ws_bytes,
                        int64_t* offsets_d, int64_t n_rec, void* stream);
 
+/*
+ * Record chains of many spans of one buffer in one launch sequence: the records of
+ * span s are data_d[span_start_d[s] : span_end_d[s]), both ends record boundaries
+ * (a batch of region requests, each inflated from a block where a record starts
+ * and cut where the next request's first record would start). seg_base_d[s] is
+ * the first segment of span s (seg_base_d[s + 1] - seg_base_d[s] =
+ * ceil(span length / seg_bytes); n_seg = seg_base_d[n_spans]); span_nref_d[s] is
+ * the reference count of the file span s comes from. Writes the result
+ * slots (CBM_REC_TAIL unused; a span that does not end exactly at its end counts
+ * in CBM_REC_INVALID) and span_rec_d[s] = index of the first record of span s,
+ * span_rec_d[n_spans] = all records. No counterpart in the reference, which
+ * decodes whole files (src/cubam/bam.py:421-446 is the single-span case).
+ */
+int64_t cbm_record_spans_workspace(int64_t n_seg);
+int cbm_record_chain_spans(const uint8_t* data_d, const int64_t* span_start_d, const int64_t* span_end_d,
+                           const int64_t* seg_base_d, const int32_t* span_nref_d, int64_t n_spans, int64_t n_seg,
+                           int64_t* result_d, int64_t* span_rec_d, void* ws_d, int64_t ws_bytes, void* stream);
+/* Repair of cbm_record_chain_spans as cbm_record_chain_repair (synchronises the stream). */
+int cbm_record_chain_spans_repair(const uint8_t* data_d, const int64_t* span_start_d, const int64_t* span_end_d,
+                                  const int64_t* seg_base_d, const int32_t* span_nref_d, int64_t n_spans,
+                                  int64_t n_seg, int64_t* result_d, int64_t* span_rec_d, void* ws_d, int64_t ws_bytes,
+                                  void* stream);
+/* Offsets (from data start) of the n_rec records of all spans, span by span. */
+int cbm_record_offsets_spans(const uint8_t* data_d, int64_t n_seg, const void* ws_d, int64_t ws_bytes,
+                             int64_t* offsets_d, int64_t n_rec, void* stream);
+
 /* ------------------------------------------------------------------ column scatter (kernel 4) */
 
 /* Column outputs of cbm_scatter (device pointers; a NULL pointer skips that column). */
