@@ -10,7 +10,10 @@ chain and the column scatter (fixed fields, CIGAR/seq/qual CSR, CB/UB tags).
 Prints one JSON line. `value` is inflated GB/s (U / step time); the roofline
 entry is for the inflate kernel, with algorithmic bytes C + U per launch; the
 CPU baseline is the oracle (zlib inflate + CRC check, record walk, columns) on
-all host cores over a bounded sample of the same chunk.
+all host cores over a bounded sample of the same chunk. The `serving` entry
+compares the served request loop (continuous batching, HBM budget, resident
+results) with plain one-request-per-call decode on the same request traces:
+closed-loop throughput and p50/p99 latency at swept Poisson arrival rates.
 """
 
 from __future__ import annotations
@@ -103,6 +106,73 @@ def cpu_baseline(chunk, header_size: int, n_ref: int, budget_s: float = 12.0):
         f" inflated), median of {reps}"
 
 
+SERVE_FILES = ((0, 30), (0, 31), (0, 32), (0, 33), (1, 34), (1, 35), (1, 36), (1, 37))
+
+
+def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, budget_gb: float = 64.0):
+    """Served vs plain region decode over 8 files held in pinned host memory.
+
+    Returns the JSON object of the serving comparison: closed-loop (every request
+    queued at once) throughput of both modes, and p50/p99 latency at Poisson
+    arrival rates given as multiples of the plain mode's closed-loop request rate.
+    """
+    import torch
+
+    from cubam_amd import synth
+    from cubam_amd.serve import BamSource, Server, make_trace, run_trace, summarize
+
+    t = time.time()
+    srcs = []
+    for kind, seed in SERVE_FILES:
+        n = gex_records if kind == 0 else atac_records
+        b = synth.make_bam(kind, seed, n, n_total=n)
+        srcs.append(BamSource(b.bgzf, name=f"{'GEX' if kind == 0 else 'ATAC'}-{seed}"))
+        del b
+    c_total = sum(s.size for s in srcs)
+    u_total = sum(s.index.stream_bytes for s in srcs)
+    log(f"serving: 8 files, {c_total / 1e9:.2f} GB compressed, {u_total / 1e9:.2f} GB inflated,"
+        f" generated and indexed in {time.time() - t:.1f} s")
+    budget = int(budget_gb * 1e9)
+
+    def run(mode, trace):
+        with Server(srcs, budget_bytes=budget, mode=mode) as srv:
+            reqs = run_trace(srv, trace)
+            out = summarize(reqs)
+            out["peak_device_GB"] = round(srv.peak_bytes / 1e9, 3)
+        torch.cuda.synchronize()
+        if out["errors"]:
+            raise RuntimeError(f"{mode}: {out['errors']} requests failed")
+        return out
+
+    warm = make_trace(srcs, 24, 0.0, seed=99)
+    run("plain", warm)
+    run("served", warm)
+    closed = make_trace(srcs, n_requests, 0.0, seed=1)
+    plain = run("plain", closed)
+    served = run("served", closed)
+    log(f"closed loop: plain {plain}; served {served}")
+    plain_rps = plain["requests"] / plain["wall_s"]
+    sweep = []
+    for i, m in enumerate(rates):
+        tr = make_trace(srcs, n_requests, m * plain_rps, seed=10 + i)
+        row = {"rate_rps": round(m * plain_rps, 2), "rate_x_plain_capacity": m,
+               "plain": run("plain", tr), "served": run("served", tr)}
+        log(f"rate {row['rate_rps']}/s: plain p50 {row['plain']['p50_ms']} p99 {row['plain']['p99_ms']} ms;"
+            f" served p50 {row['served']['p50_ms']} p99 {row['served']['p99_ms']} ms")
+        sweep.append(row)
+    return {
+        "workload": (f"8 synthetic BAMs in pinned host memory (4 GEX-like x {gex_records} records, 4 ATAC-like x"
+                     f" {atac_records}); region requests, compressed span log-uniform 64 KiB-256 MiB"),
+        "compressed_bytes": c_total,
+        "inflated_bytes": u_total,
+        "budget_GB": budget_gb,
+        "requests_per_run": n_requests,
+        "closed_loop": {"plain": plain, "served": served,
+                        "served_over_plain_GBps": round(served["GBps"] / plain["GBps"], 3)},
+        "sweep": sweep,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +180,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chunk-mib", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-gex-records", type=int, default=2_000_000)
+    ap.add_argument("--serve-atac-records", type=int, default=4_000_000)
+    ap.add_argument("--serve-requests", type=int, default=160)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -189,6 +263,10 @@ def main():
             "launch_ms": round(inf_ms, 3),
         },
     }
+    del buf
+    if not args.no_serve:
+        res["serving"] = serve_bench(args.serve_gex_records, args.serve_atac_records, args.serve_requests,
+                                     rates=(0.5, 1.0, 2.0))
     if not args.no_cpu:
         v, cores, sample = cpu_baseline(chunk, hdr.size, n_ref)
         res["cpu_baseline"] = {"value": round(v, 3), "unit": "GB/s", "cores": cores, "kind": "port",
