@@ -73,6 +73,25 @@ def gpu_step(buf, n, start, n_ref, ws, ev=None):
     return inf, chain, off, cols
 
 
+def h2d_time(chunk, buf, reps: int = 3) -> float:
+    """Milliseconds to copy the chunk from pinned host memory into HBM (median of reps)."""
+    import numpy as np
+    import torch
+
+    host = torch.empty(chunk.size, dtype=torch.uint8, pin_memory=True)
+    host.numpy()[:] = chunk
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(reps):
+        e0.record()
+        buf[: chunk.size].copy_(host, non_blocking=True)
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    del host
+    return float(np.median(times))
+
+
 def column_bytes(chain) -> int:
     """K: bytes of the column schema written (offset 8 + fixed 35 + CSR + 2 tags x 21 per record)."""
     n = chain.n_rec
@@ -205,6 +224,7 @@ def main():
     ws = _decode.Workspace()
     buf = _decode.to_device(chunk)
     n = int(chunk.size)
+    h2d_ms = h2d_time(chunk, buf)
     ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
     for _ in range(args.warmup):
         out = gpu_step(buf, n, hdr.size, n_ref, ws)
@@ -251,6 +271,8 @@ def main():
             "p99_step_ms": round(float(np.percentile(step_ms, 99)), 3),
             "path_GBps_C_U_K": round((n + U + K) / (ms / 1e3) / 1e9, 2),
             "inflate_tune": list(_decode.get_inflate_tune()),
+            "h2d_ms": round(h2d_ms, 3),
+            "pcie_inclusive_GBps": round(U / ((ms + h2d_ms) / 1e3) / 1e9, 3),
         },
         "roofline": {
             "bound": "hbm",

@@ -2,23 +2,28 @@
 // threads at once, output staged in LDS, CRC32 fused.
 //
 // DEFLATE is serial within a Huffman block, but Huffman codes resynchronise: a
-// decoder started at an arbitrary bit soon falls onto the true symbol boundaries.
-// Each round of a Huffman block gives thread t the SEG bits that start at
-// base + t * SEG. Pass A decodes speculatively from there to the first symbol that
-// starts at or after the next segment, and remembers every symbol start it saw.
-// The walks are then linked: thread t's true entry is thread t-1's exit; a thread
-// whose entry changed decodes again from it until it lands on a start it already
-// saw (from there its first walk was right) — repeated until nothing changes.
-// Pass B counts each thread's output bytes, a workgroup prefix sum places them,
-// and pass C writes literals and copies matches in LDS; a match whose source lies
-// in a segment still being written waits on that segment's progress counter
-// (the earliest waiting match always has its source written, so this ends).
+// decoder started at an arbitrary bit soon falls onto the true symbol boundaries
+// (the speculative self-synchronising decode of Weissenberger and Schmidt,
+// "Massively Parallel Huffman Decoding on GPUs", ICPP 2018). Each round of a
+// Huffman block gives thread t the SEG bits that start at base + t * SEG.
+// Pass A decodes speculatively from there to the first symbol that starts at or
+// after the next segment, and remembers every symbol start it saw. The walks are
+// then linked: thread t's true entry is thread t-1's exit; a thread whose entry
+// changed decodes again from it until it lands on a start it already saw (from
+// there its first walk was right) - repeated until nothing changes.
+// Pass B counts each thread's output bytes and matches, a workgroup prefix sum
+// places them, and pass C writes the literals into the LDS output window and the
+// matches, in output order, into a per-workgroup list in global scratch. Pass D
+// applies the matches frontier by frontier: with F the destination of the first
+// match not yet applied, every byte before F is final, so every match whose
+// source ends at or before F is applied at once, and F moves on.
 // The block's output then leaves LDS in one coalesced store, and its CRC32 is
-// computed from LDS by all threads (slices combined with x^(8n) mod P).
+// computed from LDS by all threads (slices combined with x^(8n) mod P, the
+// method of zlib's crc32_combine).
 //
-// Global memory is touched only by the coalesced staging of each round's
-// compressed bits into LDS and by the final store: no load ever waits behind the
-// decoder's own stores.
+// Global memory is touched by the coalesced staging of each round's compressed
+// bits into LDS, the match list, and the final store; no load waits behind the
+// decoder's byte stores, which all go to LDS.
 #pragma once
 #include "inflate.h"
 
