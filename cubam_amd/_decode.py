@@ -177,6 +177,11 @@ def inflate_blocks(blocks: Blocks, prefix: torch.Tensor | None = None, ws: Works
     status = torch.empty(max(nb, 1), dtype=_I32, device=dev)
     result = torch.empty(1, dtype=_I64, device=dev)
     t = tune if tune is not None else _tune
+    if tune is None and not any(get_inflate_tune()):
+        from cubam_amd import tune as autotuner
+
+        if autotuner.enabled():
+            t = autotuner.lookup(nb, blocks.total_out) or t
     iws = ws.dev("inflate_ws", L.cbm_inflate_workspace(nb))
     check(
         L.cbm_inflate(

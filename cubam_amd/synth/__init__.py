@@ -125,22 +125,29 @@ def bgzf_blocks(
     lv = np.ascontiguousarray(np.broadcast_to(np.asarray(level, np.int8), (n,)))
     fx = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
     stride = 1 << 16
-    tmp = np.empty(n * stride, np.uint8)
+    batch = min(n, 16384)  # blocks compressed per call: a bounded staging buffer
+    tmp = np.empty(batch * stride, np.uint8)
     bsize = np.empty(n, np.int32)
     data = np.ascontiguousarray(data, np.uint8)
-    rc = lib().cbs_bgzf_blocks(
-        _ptr(data), _ptr(off), _ptr(size), n, _ptr(lv), _ptr(fx), _ptr(tmp), stride, _ptr(bsize),
-        n_threads or default_threads(),
-    )
-    if rc != 0:
-        raise RuntimeError("zlib failed")
-    total = int(bsize.sum(dtype=np.int64))
-    out = np.empty(total, np.uint8)
-    pos = 0
-    for i in range(n):
-        b = int(bsize[i])
-        out[pos : pos + b] = tmp[i * stride : i * stride + b]
-        pos += b
+    parts = []
+    for b0 in range(0, n, batch):
+        b1 = min(n, b0 + batch)
+        rc = lib().cbs_bgzf_blocks(
+            _ptr(data), _ptr(off[b0:b1]), _ptr(size[b0:b1]), b1 - b0, _ptr(lv[b0:b1]),
+            None if fx is None else _ptr(fx[b0:b1]), _ptr(tmp), stride, _ptr(bsize[b0:b1]),
+            n_threads or default_threads(),
+        )
+        if rc != 0:
+            raise RuntimeError("zlib failed")
+        bs = bsize[b0:b1]
+        part = np.empty(int(bs.sum(dtype=np.int64)), np.uint8)
+        pos = 0
+        for i in range(b1 - b0):
+            k = int(bs[i])
+            part[pos : pos + k] = tmp[i * stride : i * stride + k]
+            pos += k
+        parts.append(part)
+    out = np.concatenate(parts) if len(parts) > 1 else parts[0]
     return out, bsize
 
 

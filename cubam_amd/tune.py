@@ -1,0 +1,199 @@
+"""Autotuner of the inflate launch shape, and a hipGraph of a batched inflate.
+
+The inflate kernel comes in launch shapes (``cbm_inflate_tune``):
+
+* workgroup variants, where a workgroup of NT threads decodes one block at a time
+  with SEG bits per thread and round. NT and SEG set the LDS layout: the
+  compressed-bit staging area is NT * SEG / 8 bytes beside the 64 KiB output
+  window and the decode tables.
+* the lane variant, where every lane of a wavefront inflates a block of its own,
+  with ``blocks_per_wave`` x ``waves_per_group`` blocks per workgroup.
+
+:func:`autotune` times every candidate on a sample of a batch's blocks and keeps
+the fastest per (GPU, batch-size bucket) in a JSON cache
+(``$CUBAM_AMD_TUNE_CACHE``, default ``~/.cache/cubam_amd/inflate_tune.json``).
+:func:`lookup` returns the cached choice for a batch, and
+:func:`cubam_amd._decode.inflate_blocks` uses it once :func:`enable` has been
+called (or ``CUBAM_AMD_AUTOTUNE=1`` is set).
+
+:class:`GraphInflate` captures the inflate of one fixed set of blocks (a
+batched heterogeneous decode) in a hipGraph, so each later decode of new bytes
+in the same buffers is one graph launch.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+from pathlib import Path
+
+import torch
+
+from cubam_amd import _decode, _lib
+
+#: workgroup variants, then the lane variant at 1, 2, 4 and 8 blocks per workgroup
+CANDIDATES = tuple(
+    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8)]
+    + [(_lib.INFLATE_LANES, bpw, 1, 0) for bpw in (1, 2, 4, 8)]
+    + [(_lib.INFLATE_LANES, 4, 2, 0), (_lib.INFLATE_LANES, 8, 2, 0)]
+)
+
+_enabled = os.environ.get("CUBAM_AMD_AUTOTUNE", "") not in ("", "0")
+_cache: dict | None = None
+
+
+def cache_path() -> Path:
+    p = os.environ.get("CUBAM_AMD_TUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "cubam_amd" / "inflate_tune.json"
+
+
+def enable(on: bool = True) -> None:
+    """Let inflate launches use the cached autotuned shape for their bucket."""
+    global _enabled
+    _enabled = on
+
+
+def enabled() -> bool:
+    return _enabled
+
+
+def _load() -> dict:
+    global _cache
+    if _cache is None:
+        try:
+            _cache = json.loads(cache_path().read_text())
+        except (OSError, ValueError):
+            _cache = {}
+    return _cache
+
+
+def _save() -> None:
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        tmp = p.with_suffix(f".tmp{os.getpid()}")
+        tmp.write_text(json.dumps(_load(), indent=1, sort_keys=True))
+        os.replace(tmp, p)
+    except OSError:
+        pass  # a read-only home: the choice stays in this process
+
+
+def device_key() -> str:
+    prop = torch.cuda.get_device_properties(torch.cuda.current_device())
+    arch = getattr(prop, "gcnArchName", prop.name).split(":")[0]
+    return f"{arch}/{prop.multi_processor_count}cu"
+
+
+def bucket(n_blocks: int, inflated: int) -> str:
+    """Batch-size bucket: powers of two of the block count and of the mean block size."""
+    nb = max(1, n_blocks)
+    mean = max(1, inflated // nb)
+    return f"b{int(math.log2(nb))}-m{int(math.log2(mean))}"
+
+
+def lookup(n_blocks: int, inflated: int) -> _lib.InflateTune | None:
+    """The autotuned shape for a batch, if one is cached for its bucket on this GPU."""
+    e = _load().get(device_key(), {}).get(bucket(n_blocks, inflated))
+    return _lib.InflateTune(*e["tune"]) if e else None
+
+
+def record(n_blocks: int, inflated: int, tune, ms: float, table: dict | None = None) -> None:
+    d = _load().setdefault(device_key(), {})
+    d[bucket(n_blocks, inflated)] = {"tune": list(tune), "ms": ms, "candidates": table or {}}
+    _save()
+
+
+def time_inflate(blocks: _decode.Blocks, tune, reps: int = 2, ws: _decode.Workspace | None = None) -> float:
+    """Milliseconds of one inflate launch over ``blocks`` with shape ``tune`` (best of reps)."""
+    t = _lib.InflateTune(*tune)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(reps):
+        ev0.record()
+        inf = _decode.inflate_blocks(blocks, None, ws, tune=t)
+        ev1.record()
+        ev1.synchronize()
+        _decode.check_inflated(inf)
+        best = min(best, ev0.elapsed_time(ev1))
+        del inf
+    return best
+
+
+def sample_blocks(blocks: _decode.Blocks, n: int) -> _decode.Blocks:
+    """The first n blocks of a scan, as a batch of their own."""
+    n = min(n, blocks.n_blocks)
+    out_off = blocks.out_off[: n + 1]
+    return _decode.Blocks(blocks.buf, n, int(blocks.blk_off[n - 1] + blocks.bsize[n - 1]) if n else 0,
+                          int(out_off[-1]), blocks.blk_off[:n], blocks.bsize[:n], blocks.isize[:n],
+                          blocks.crc[:n], out_off)
+
+
+def autotune(blocks: _decode.Blocks, sample: int = 50_000, candidates=CANDIDATES, log=None,
+             ws: _decode.Workspace | None = None) -> tuple[tuple, dict]:
+    """Time every candidate shape on the first ``sample`` blocks; cache and return the fastest.
+
+    Returns ``(best_tune, {candidate: ms})``. The cache entry is keyed by the
+    bucket of the whole batch, so later batches of that size use it.
+    """
+    s = sample_blocks(blocks, sample)
+    table = {}
+    t0 = time.time()
+    for c in candidates:
+        ms = time_inflate(s, c, ws=ws)
+        table[",".join(map(str, c))] = round(ms, 3)
+        if log:
+            log(f"autotune {c}: {ms:.2f} ms for {s.n_blocks} blocks")
+    best = min(candidates, key=lambda c: table[",".join(map(str, c))])
+    record(blocks.n_blocks, blocks.total_out, best, table[",".join(map(str, best))], table)
+    if log:
+        log(f"autotune: best {best} in {time.time() - t0:.1f} s")
+    return best, table
+
+
+class GraphInflate:
+    """The inflate of a fixed block table, captured once in a hipGraph and replayed.
+
+    The compressed buffer, the block table and the outputs stay at fixed device
+    addresses: write new compressed bytes of the same block layout into
+    ``blocks.buf`` and call :meth:`run`. The graph holds only kernels (the
+    workspace counters are cleared by a fill kernel, not a memset).
+    """
+
+    def __init__(self, blocks: _decode.Blocks, tune=None, ws: _decode.Workspace | None = None):
+        L = _lib.load()
+        self.blocks = blocks
+        self.tune = _lib.InflateTune(*(tune or _decode.get_inflate_tune()))
+        dev = blocks.buf.device
+        nb = blocks.n_blocks
+        self.data = torch.empty(blocks.total_out + _lib.PAD, dtype=torch.uint8, device=dev)
+        self.crc = torch.empty(max(nb, 1), dtype=torch.uint32, device=dev)
+        self.status = torch.empty(max(nb, 1), dtype=torch.int32, device=dev)
+        self.bad = torch.empty(1, dtype=torch.int64, device=dev)
+        self.ws = torch.empty(L.cbm_inflate_workspace(nb), dtype=torch.uint8, device=dev)
+        self._launch()  # eager once: kernel attributes are set outside the capture
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._launch()
+        torch.cuda.synchronize()
+
+    def _launch(self) -> None:
+        import ctypes
+
+        L = _lib.load()
+        b = self.blocks
+        _lib.check(L.cbm_inflate(b.buf.data_ptr(), b.blk_off.data_ptr(), b.bsize.data_ptr(), b.isize.data_ptr(),
+                                 b.crc.data_ptr(), b.out_off.data_ptr(), b.n_blocks, self.data.data_ptr(),
+                                 self.crc.data_ptr(), self.status.data_ptr(), self.bad.data_ptr(),
+                                 ctypes.byref(self.tune), self.ws.data_ptr(), self.ws.numel(), _lib.stream_ptr()),
+                   "cbm_inflate")
+
+    def run(self) -> None:
+        """Replay the captured launch on the current stream (asynchronous)."""
+        self.graph.replay()
+
+    def check(self) -> None:
+        """Synchronise and raise if any block did not inflate cleanly."""
+        _decode.raise_block_errors(self, int(self.bad.item()))
