@@ -92,6 +92,18 @@ def h2d_time(chunk, buf, reps: int = 3) -> float:
     return float(np.median(times))
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 3) -> None:
+    """A plain device-to-device copy of known size, for scaling the HBM byte counters."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1, out=dst)  # an elementwise kernel (a plain copy_ may go to a DMA engine)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
+
+
 def column_bytes(chain) -> int:
     """K: bytes of the column schema written (offset 8 + fixed 35 + CSR + 2 tags x 21 per record)."""
     n = chain.n_rec
@@ -259,6 +271,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--batched-blocks", type=int, default=500_000)
     ap.add_argument("--serve-gex-records", type=int, default=2_000_000)
     ap.add_argument("--serve-atac-records", type=int, default=4_000_000)
@@ -273,6 +287,10 @@ def main():
 
     from cubam_amd import _decode, synth
     from cubam_amd._lib import load
+
+    if args.calibrate_copy:
+        copy_calibration()
+        return
 
     load()
     synth.build()
@@ -341,7 +359,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": profiled_traffic(_kernel_name()),
+            "traffic_source": "profiles/hbm_traffic_atac_1GiB_chunk.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
+            "algorithmic_bytes": n + U,
             "launch_ms": round(inf_ms, 3),
         },
     }
@@ -356,6 +376,21 @@ def main():
         res["cpu_baseline"] = {"value": round(v, 3), "unit": "GB/s", "cores": cores, "kind": "port",
                                "sample": sample}
     print(json.dumps(res), flush=True)
+
+
+def profiled_traffic(shape_label: str):
+    """HBM bytes per launch of the inflate kernel from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy kernel), if it is for the same kernel shape; else None."""
+    p = ROOT / "profiles" / "hbm_traffic_atac_1GiB_chunk.md"
+    try:
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic_per_launch_bytes "):
+                name, value = line[len("traffic_per_launch_bytes "):].rsplit(" ", 1)
+                if name in shape_label:
+                    return int(value)
+    except OSError:
+        pass
+    return None
 
 
 def _kernel_name() -> str:
