@@ -22,11 +22,11 @@ BLOCK_OK, BLOCK_DEFLATE, BLOCK_SIZE, BLOCK_CRC = 0, 1, 2, 3
 REC_N, REC_TAIL, REC_INVALID, REC_CIGAR, REC_SEQ, REC_QUAL, REC_BAD_LINKS, REC_NSEG = range(8)
 
 
-INFLATE_WG256, INFLATE_LANES, INFLATE_WG128, INFLATE_WG512, INFLATE_WG64 = 0, 1, 2, 3, 4
-INFLATE_VARIANTS = 9
+INFLATE_WG512_S128, INFLATE_LANES, INFLATE_WG128, INFLATE_WG512, INFLATE_WG64 = 0, 1, 2, 3, 4
+INFLATE_VARIANTS = 11
 #: (threads per block, segment bits) of each workgroup variant; None = one block per lane
-INFLATE_SHAPES = {0: (256, 256), 1: None, 2: (128, 256), 3: (512, 64), 4: (64, 512), 5: (256, 128), 6: (128, 128),
-                  7: (64, 128), 8: (64, 256)}
+INFLATE_SHAPES = {0: (512, 128), 1: None, 2: (128, 256), 3: (512, 64), 4: (64, 512), 5: (256, 128), 6: (128, 128),
+                  7: (64, 128), 8: (64, 256), 9: (384, 128), 10: (256, 256)}
 
 
 class InflateTune(ctypes.Structure):

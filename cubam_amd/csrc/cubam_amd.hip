@@ -295,6 +295,7 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
     case V:                                                                                                   \
         return launch_wave<NT_, SEG_>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, \
                                       result, groups, counter, st);
+            WAVE_CASE(CBM_INFLATE_WG512_S128, 512, 128)
             WAVE_CASE(CBM_INFLATE_WG256, 256, 256)
             WAVE_CASE(CBM_INFLATE_WG128, 128, 256)
             WAVE_CASE(CBM_INFLATE_WG512, 512, 64)
@@ -303,6 +304,7 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
             WAVE_CASE(CBM_INFLATE_WG128_S128, 128, 128)
             WAVE_CASE(CBM_INFLATE_WG64_S128, 64, 128)
             WAVE_CASE(CBM_INFLATE_WG64_S256, 64, 256)
+            WAVE_CASE(CBM_INFLATE_WG384_S128, 384, 128)
 #undef WAVE_CASE
             default: ARG(false, "cbm_inflate: unknown variant");
         }

@@ -32,21 +32,28 @@ namespace cbm {
 enum { SV_POS = 0, SV_FINAL, SV_TYPE, SV_ERR, SV_LEN, SV_SRC, SV_OP, SV_CLEAR, SV_TEND, SV_CHANGED, SV_DONE, SV_OK,
        SV_BLOCK, SV_MAX, SV_SITE, SV_F0, SV_F1, SV_F2, SV_N };
 
-// matches each thread holds in pass D; copies move MD_PIECE bytes per round trip
-constexpr int MD_K = 8;
-constexpr int MD_PIECE = 16;
+// matches each thread holds in pass D (fewer in the widest workgroups, to save registers)
+template <int NT>
+constexpr int md_k() { return NT >= 512 ? 4 : 8; }
 
 template <int NT, int SEG>
 struct WaveSmem {
     static constexpr int IN_WORDS = NT * SEG / 32 + 8;  // staged compressed bits of a round
+    static_assert(IN_WORDS >= 256, "the staging area holds the CRC32 byte table at block end");
     uint8_t out[65536 + 16];
     uint32_t in[IN_WORDS];
     uint16_t lit[LTAB];
     uint16_t dist[DTAB];
-    uint16_t code[320];
-    uint8_t lens[320];
-    int32_t exit_[NT];
-    int32_t stop_[NT];
+    union {
+        struct {  // header and table build
+            uint16_t code[320];
+            uint8_t lens[320];
+        };
+        struct {  // rounds: each walk's exit and how it stopped
+            int32_t exit_[NT];
+            uint8_t stop_[NT];
+        };
+    };
     int32_t red[NT / CBM_WAVE];
     int64_t red64[NT / CBM_WAVE];
     int32_t lcount[16];
@@ -62,7 +69,11 @@ enum { SITE_HEADER = 1, SITE_STORED = 2, SITE_TYPE = 3, SITE_CLCODE = 4, SITE_LE
 
 enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 
-// decoder statistics (cycles of thread 0 per phase, and rounds), summed over blocks
+// decoder statistics (cycles of thread 0 per phase, and rounds), summed over blocks, in builds with
+// CBM_INFLATE_STATS=1 (they cost registers)
+#ifndef CBM_INFLATE_STATS
+#define CBM_INFLATE_STATS 0
+#endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
        STAT_DSTEPS, STAT_LINKIT, STAT_N };
 
@@ -310,6 +321,79 @@ __device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, uint16
     return true;
 }
 
+// ------------------------------------------------------------------ match copies in LDS
+// Copy n bytes from s to d (s + n <= d: no overlap) in the output window: bytes up to a 4-byte
+// boundary of d, then whole aligned words (the source read as aligned words and realigned
+// with v_alignbyte), then the last bytes. Source words may hold bytes outside the range that
+// other threads are writing: those bytes are shifted out, never used.
+__device__ __forceinline__ void copy_bytes(uint8_t* out, int d, int s, int n) {
+    while (n > 0 && (d & 3)) {
+        out[d++] = out[s++];
+        n--;
+    }
+    if (n >= 4) {
+        const uint32_t* w = (const uint32_t*)out;
+        uint32_t* wd = (uint32_t*)out;
+        const int sh = s & 3;
+        if (sh == 0) {
+            for (; n >= 4; n -= 4, d += 4, s += 4) wd[d >> 2] = w[s >> 2];
+        } else {
+            uint32_t lo = w[s >> 2];
+            for (; n >= 4; n -= 4, d += 4, s += 4) {
+                const uint32_t hi = w[(s >> 2) + 1];
+                wd[d >> 2] = __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)sh);
+                lo = hi;
+            }
+        }
+    }
+    while (n > 0) {
+        out[d++] = out[s++];
+        n--;
+    }
+}
+
+// Apply a match: len bytes at dest copied from dist bytes back (RFC 1951 3.2.3 semantics: an
+// overlapping copy repeats the dist bytes before dest)
+__device__ __forceinline__ void lds_copy_match(uint8_t* out, int dest, int dist, int len) {
+    const int src = dest - dist;
+    if (dist >= len) {
+        copy_bytes(out, dest, src, len);
+    } else if (dist >= 8) {
+        // dest + k * dist .. : each piece repeats [src, src + dist), which is final
+        for (int k = 0; k < len; k += dist) copy_bytes(out, dest + k, src, len - k < dist ? len - k : dist);
+    } else {
+        // the dist bytes before dest repeat: read them once, then only write
+        uint64_t pat = 0;
+        for (int q = 0; q < dist; q++) pat |= (uint64_t)out[src + q] << (8 * q);
+        int r = 0;
+        for (int q = 0; q < len; q++) {
+            out[dest + q] = (uint8_t)(pat >> (8 * r));
+            r = r + 1 == dist ? 0 : r + 1;
+        }
+    }
+}
+
+// Set (or clear) the bits [d, d + n) of a bitmap in LDS, a word at a time.
+__device__ __forceinline__ void mark_bits(uint32_t* bm, int d, int n, bool set) {
+    const int e = d + n;
+    for (int w = d >> 5; w <= (e - 1) >> 5; w++) {
+        const int a = w * 32 > d ? w * 32 : d, b = w * 32 + 32 < e ? w * 32 + 32 : e;
+        const uint32_t m = (b - a == 32 ? 0xffffffffu : ((1u << (b - a)) - 1u)) << (a & 31);
+        if (set) atomicOr(&bm[w], m);
+        else atomicAnd(&bm[w], ~m);
+    }
+}
+
+// Any bit of [a, b) set?
+__device__ __forceinline__ bool any_bits(const uint32_t* bm, int a, int b) {
+    for (int w = a >> 5; w <= (b - 1) >> 5; w++) {
+        const int x = w * 32 > a ? w * 32 : a, y = w * 32 + 32 < b ? w * 32 + 32 : b;
+        const uint32_t m = (y - x == 32 ? 0xffffffffu : ((1u << (y - x)) - 1u)) << (x & 31);
+        if (__atomic_load_n(&bm[w], __ATOMIC_RELAXED) & m) return true;
+    }
+    return false;
+}
+
 // ------------------------------------------------------------------ staging
 // Stage payload words [w0, w0 + IN_WORDS) (word = 4 payload bytes; payload byte 0 = in[0])
 // into LDS; bytes at or past `limit` (payload bytes + trailer) read as zero.
@@ -333,8 +417,10 @@ __device__ void stage(S& s, const uint8_t* in, int w0, int limit) {
 }
 
 // ------------------------------------------------------------------ the kernel
+// two workgroups share a CU (the LDS holds two output windows): ask for the registers that let
+// both run, NT / 128 wavefronts per SIMD
 template <int NT, int SEG>
-__global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int64_t* blk_off,
+__global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(const uint8_t* src, const int64_t* blk_off,
                                                    const int32_t* bsize, const int32_t* isize,
                                                    const uint32_t* expect, const int64_t* out_off,
                                                    int64_t n_blocks, uint8_t* dst, uint32_t* crc_out,
@@ -344,9 +430,9 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
     using S = WaveSmem<NT, SEG>;
     extern __shared__ __align__(16) uint8_t smem_raw[];
     S& s = *(S*)smem_raw;
-    __shared__ uint32_t crcT[256];
     const int tid = threadIdx.x;
     uint64_t* mlist = mlists + (size_t)blockIdx.x * MLIST_CAP;
+#if CBM_INFLATE_STATS
     // cycles per phase, kept by thread 0 and added to stats[] once per block
     long long acc[STAT_N] = {};
     long long tlast = tid == 0 ? clock64() : 0;
@@ -358,13 +444,21 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
             tlast = t_;                   \
         }                                 \
     } while (0)
-    // CRC tables and this thread's slice factor x^(8 * slice * (NT - 1 - tid)) mod P
-    constexpr int SLICE = 65536 / NT;
-    for (int i = tid; i < 256; i += NT) {
-        uint32_t c = i;
-        for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-        crcT[i] = c;
-    }
+#define COUNT(k)                \
+    do {                        \
+        if (tid == 0) acc[k]++; \
+    } while (0)
+#else
+#define PHASE(k) \
+    do {         \
+    } while (0)
+#define COUNT(k) \
+    do {         \
+    } while (0)
+    (void)stats;
+#endif
+    // this thread's CRC32 slice factor x^(8 * SLICE * (NT - 1 - tid)) mod P
+    constexpr int SLICE = (65536 + NT - 1) / NT;
     uint32_t xfac;
     {
         uint32_t r = 0x80000000u, sq = 0x40000000u;  // x^0, x^1
@@ -473,7 +567,7 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 stage<NT>(s, in, wb1, limit);
                 __syncthreads();
                 if (tid == 0) {
-                    int p = p1, n = 0, total = hlit + hdist, err = 0;
+                    int p = p1, n = 0, total = hlit + hdist, err = 0, prev = -1;
                     int wb = wb1;
                     uint64_t v = 0;
                     int avail = 0;
@@ -491,12 +585,13 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                         avail -= l;
                         if (sym < 16) {
                             s.lens[n++] = (uint8_t)sym;
+                            prev = sym;
                         } else {
                             int rep, xb;
                             uint8_t val = 0;
                             if (sym == 16) {
-                                if (n == 0) { err = 1; break; }
-                                val = s.lens[n - 1];
+                                if (prev < 0) { err = 1; break; }
+                                val = (uint8_t)prev;
                                 rep = 3 + (int)(v & 3);
                                 xb = 2;
                             } else if (sym == 17) {
@@ -511,6 +606,7 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                             avail -= xb;
                             if (n + rep > total) { err = 1; break; }
                             while (rep--) s.lens[n++] = val;
+                            prev = val;
                         }
                         if (p > in_bits) { err = 1; break; }
                     }
@@ -623,7 +719,7 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                     }
                     __syncthreads();
                     const int changed = s.sv[SV_F0 + it % 3];
-                    if (tid == 0) acc[STAT_LINKIT]++;
+                    COUNT(STAT_LINKIT);
                     if (!changed) break;
                 }
                 PHASE(STAT_LINK);
@@ -692,12 +788,16 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 __syncthreads();
                 if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
                 PHASE(STAT_PASS_C);
-                // pass D: apply the round's matches. Let F be the destination of the first match
-                // not yet applied: every byte before F is final, so every match whose source ends
-                // at or before F can be applied now, wherever it is in the list. Each step applies
-                // all of those at once (each thread holds up to MD_K matches of the list, in
-                // order) and F moves on to the first one left. Rounds with more matches than the
-                // threads hold go through one wavefront in list order.
+                // pass D: apply the round's matches. A match can be copied once the bytes it reads
+                // are final. Variants whose staging area holds a bit per output byte (64 Ki bits)
+                // keep the round's pending output bytes there: each step, every thread applies each
+                // of its matches (up to MD_K, in list order) whose source bytes are no longer
+                // pending, then clears its destination bits. The first unapplied match always has
+                // its source final (sources precede destinations), so every step applies one at
+                // least. Other variants use the frontier F, the destination of the first unapplied
+                // match: every byte before F is final. Rounds with more matches than the threads
+                // hold go through one wavefront in list order.
+                constexpr int MD_K = md_k<NT>();
                 if (mtotal <= MD_K * NT) {
                     uint64_t mm[MD_K];
                     uint32_t left = 0;
@@ -707,56 +807,70 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                         mm[u] = k < mtotal ? __atomic_load_n(&mlist[k], __ATOMIC_RELAXED) : 0;
                         if (k < mtotal) left |= 1u << u;
                     }
-                    if (tid == 0) {
-                        s.sv[SV_F0] = mtotal ? (int)(mm[0] & 0x1ffff) : 0x7fffffff;
-                        s.sv[SV_F0 + 1] = 0x7fffffff;
-                        s.sv[SV_F0 + 2] = 0x7fffffff;
-                    }
-                    __syncthreads();
-                    for (int step = 0;; step++) {
-                        const int F = s.sv[SV_F0 + step % 3];
-                        if (F == 0x7fffffff) break;
-#pragma unroll
-                        for (int u = 0; u < MD_K; u++) {
-                            if (!(left & (1u << u))) continue;
-                            const uint64_t m = mm[u];
-                            const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
-                            const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
-                            if (src + (dist < len ? dist : len) > F) continue;
-                            left &= ~(1u << u);
-                            if (dist >= len) {
-                                for (int lo = 0; lo < len; lo += MD_PIECE) {
-                                    uint8_t t16[MD_PIECE];
-#pragma unroll
-                                    for (int q = 0; q < MD_PIECE; q++) t16[q] = lo + q < len ? s.out[src + lo + q] : 0;
-#pragma unroll
-                                    for (int q = 0; q < MD_PIECE; q++)
-                                        if (lo + q < len) s.out[dest + lo + q] = t16[q];
-                                }
-                            } else if (dist >= 8) {
-                                for (int q = 0; q < len; q++) s.out[dest + q] = s.out[src + q];
-                            } else {
-                                // the dist bytes before dest repeat: read them once, then only write
-                                uint64_t pat = 0;
-                                for (int q = 0; q < dist; q++) pat |= (uint64_t)s.out[src + q] << (8 * q);
-                                int r = 0;
-                                for (int q = 0; q < len; q++) {
-                                    s.out[dest + q] = (uint8_t)(pat >> (8 * r));
-                                    r = r + 1 == dist ? 0 : r + 1;
-                                }
-                            }
-                        }
-                        // the next frontier: a minimum within the wavefront, then one atomic per wavefront
-                        int mine = left ? (int)(mm[__ffs(left) - 1] & 0x1ffff) : 0x7fffffff;
-#pragma unroll
-                        for (int d = 32; d; d >>= 1) {
-                            int o = __shfl_xor(mine, d, CBM_WAVE);
-                            mine = o < mine ? o : mine;
-                        }
-                        if ((tid % CBM_WAVE) == 0 && mine != 0x7fffffff) atomicMin(&s.sv[SV_F0 + (step + 1) % 3], mine);
-                        if (tid == 0) s.sv[SV_F0 + (step + 2) % 3] = 0x7fffffff;
+                    if constexpr (S::IN_WORDS >= 2048 + 8) {
+                        uint32_t* pend = s.in;
+                        const int w0 = op >> 5, w1 = (op + total + 31) >> 5;
+                        for (int i = w0 + tid; i < w1; i += NT) pend[i] = 0;
+                        if (tid == 0) s.sv[SV_F0] = 0;
                         __syncthreads();
-                        if (tid == 0) acc[STAT_DSTEPS]++;
+#pragma unroll
+                        for (int u = 0; u < MD_K; u++)
+                            if (left & (1u << u)) mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), true);
+                        __syncthreads();
+                        for (int step = 0;; step++) {
+                            if (tid == 0) s.sv[SV_F0 + (step + 1) % 3] = 0;
+#pragma unroll
+                            for (int u = 0; u < MD_K; u++) {
+                                if (!(left & (1u << u))) continue;
+                                const uint64_t m = mm[u];
+                                const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+                                const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
+                                const int lo = src < op ? op : src, hi = src + (dist < len ? dist : len);
+                                if (lo < hi && any_bits(pend, lo, hi)) continue;
+                                __threadfence_block();  // read the source after its bits were seen clear
+                                left &= ~(1u << u);
+                                lds_copy_match(s.out, dest, dist, len);
+                                __threadfence_block();  // the bytes land before their bits clear
+                                mark_bits(pend, dest, len, false);
+                            }
+                            if (left) s.sv[SV_F0 + step % 3] = 1;
+                            __syncthreads();
+                            COUNT(STAT_DSTEPS);
+                            if (!s.sv[SV_F0 + step % 3]) break;
+                        }
+                    } else {
+                        if (tid == 0) {
+                            s.sv[SV_F0] = mtotal ? (int)(mm[0] & 0x1ffff) : 0x7fffffff;
+                            s.sv[SV_F0 + 1] = 0x7fffffff;
+                            s.sv[SV_F0 + 2] = 0x7fffffff;
+                        }
+                        __syncthreads();
+                        for (int step = 0;; step++) {
+                            const int F = s.sv[SV_F0 + step % 3];
+                            if (F == 0x7fffffff) break;
+#pragma unroll
+                            for (int u = 0; u < MD_K; u++) {
+                                if (!(left & (1u << u))) continue;
+                                const uint64_t m = mm[u];
+                                const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+                                const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
+                                if (src + (dist < len ? dist : len) > F) continue;
+                                left &= ~(1u << u);
+                                lds_copy_match(s.out, dest, dist, len);
+                            }
+                            // the next frontier: a minimum within the wavefront, then one atomic per wavefront
+                            int mine = left ? (int)(mm[__ffs(left) - 1] & 0x1ffff) : 0x7fffffff;
+#pragma unroll
+                            for (int d = 32; d; d >>= 1) {
+                                int o = __shfl_xor(mine, d, CBM_WAVE);
+                                mine = o < mine ? o : mine;
+                            }
+                            if ((tid % CBM_WAVE) == 0 && mine != 0x7fffffff)
+                                atomicMin(&s.sv[SV_F0 + (step + 1) % 3], mine);
+                            if (tid == 0) s.sv[SV_F0 + (step + 2) % 3] = 0x7fffffff;
+                            __syncthreads();
+                            COUNT(STAT_DSTEPS);
+                        }
                     }
                 } else if (tid < CBM_WAVE) {
                     const int lane = tid;
@@ -778,7 +892,7 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
                 }
                 __syncthreads();
                 PHASE(STAT_PASS_D);
-                if (tid == 0) acc[STAT_ROUNDS]++;
+                COUNT(STAT_ROUNDS);
                 if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
                 if (tid == 0) {
                     s.sv[SV_OP] = op + total;
@@ -798,7 +912,15 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
         const int nout = err == CBM_BLOCK_OK ? cap : (op < cap ? op : cap);
         uint8_t* d = dst + out_off[b];
         for (int i = tid; i < nout; i += NT) d[i] = s.out[i];
-        // CRC32 of the output: slice tid covers [cap - SLICE*(NT - tid), cap - SLICE*(NT-1-tid))
+        // CRC32 of the output: slice tid covers [nout - SLICE*(NT - tid), nout - SLICE*(NT-1-tid)).
+        // The byte table goes in the staging area, free once the last round is done.
+        uint32_t* crcT = s.in;
+        for (int i = tid; i < 256; i += NT) {
+            uint32_t t = i;
+            for (int k = 0; k < 8; k++) t = (t & 1) ? 0xEDB88320u ^ (t >> 1) : t >> 1;
+            crcT[i] = t;
+        }
+        __syncthreads();
         uint32_t c = 0;
         {
             int e2 = nout - SLICE * (NT - 1 - tid), s2 = e2 - SLICE;
@@ -819,13 +941,16 @@ __global__ void __launch_bounds__(NT) inflate_wave(const uint8_t* src, const int
             if (err == CBM_BLOCK_OK && x != expect[b]) err = CBM_BLOCK_CRC;
             status[b] = err | (err != CBM_BLOCK_OK ? site << 8 : 0);
             PHASE(STAT_STORE);
+#if CBM_INFLATE_STATS
             if (stats)
                 for (int k = 0; k < STAT_N; k++) atomicAdd(&stats[k], (unsigned long long)acc[k]), acc[k] = 0;
+#endif
             if (err != CBM_BLOCK_OK) atomicAdd((unsigned long long*)bad_count, 1ull);
         }
         __syncthreads();
     }
 #undef PHASE
+#undef COUNT
 }
 
 }  // namespace cbm

@@ -86,7 +86,7 @@ enum {
 /* Inflate kernel variants (cbm_inflate_tune.variant): a workgroup of NT threads decodes each
    block, each thread SEG bits of a round; or one block per lane (CBM_INFLATE_LANES). */
 enum {
-    CBM_INFLATE_WG256 = 0,      /* default: NT 256, SEG 256 */
+    CBM_INFLATE_WG512_S128 = 0, /* default: NT 512, SEG 128 */
     CBM_INFLATE_LANES = 1,      /* blocks_per_wave lanes of a wavefront each inflate a block */
     CBM_INFLATE_WG128 = 2,      /* NT 128, SEG 256 */
     CBM_INFLATE_WG512 = 3,      /* NT 512, SEG 64 */
@@ -95,7 +95,9 @@ enum {
     CBM_INFLATE_WG128_S128 = 6, /* NT 128, SEG 128 */
     CBM_INFLATE_WG64_S128 = 7,  /* NT 64, SEG 128 */
     CBM_INFLATE_WG64_S256 = 8,  /* NT 64, SEG 256 */
-    CBM_INFLATE_VARIANTS = 9
+    CBM_INFLATE_WG384_S128 = 9, /* NT 384, SEG 128 */
+    CBM_INFLATE_WG256 = 10,     /* NT 256, SEG 256 */
+    CBM_INFLATE_VARIANTS = 11
 };
 
 /* Launch shape of the inflate kernel; zeros select the defaults (or the autotuned choice). */

@@ -70,8 +70,9 @@ def test_hetero_blocks_bytes_and_crc(cbm, oracle):
 
 
 @pytest.mark.parametrize("variant,bpw,wpg,groups", [(0, 0, 0, 0), (0, 0, 0, 7), (2, 0, 0, 0), (3, 0, 0, 0),
-                                                    (4, 0, 0, 0), (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0),
-                                                    (1, 32, 1, 0)])
+                                                    (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0), (7, 0, 0, 0),
+                                                    (8, 0, 0, 0), (9, 0, 0, 0), (10, 0, 0, 0), (10, 0, 0, 3),
+                                                    (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0), (1, 32, 1, 0)])
 def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     from cubam_amd import _lib
 
