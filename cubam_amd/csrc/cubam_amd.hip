@@ -510,37 +510,43 @@ int cbm_record_offsets_spans(const uint8_t* data, int64_t n_seg, const void* ws,
 }
 
 // ------------------------------------------------------------------ kernel 4
-int64_t cbm_scatter_workspace(int64_t n_rec) { return scan_workspace(n_rec + 1) + 64; }
+// workspace: n_cigar_op and l_seq of every record (int32 each), then the scan's tile sums
+int64_t cbm_scatter_workspace(int64_t n_rec) {
+    return n_rec < 0 ? -1 : ((8 * n_rec + 255) / 256) * 256 + scan_workspace(n_rec + 1) + 64;
+}
 
 int cbm_scatter(const uint8_t* data, const int64_t* offsets, int64_t n_rec, const cbm_columns* cols, void* ws,
                 int64_t ws_bytes, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     ARG(n_rec >= 0 && cols, "cbm_scatter: bad arguments");
     ARG(cols->n_tags >= 0 && cols->n_tags <= 4, "cbm_scatter: at most 4 tags");
-    ARG(ws_bytes >= cbm_scatter_workspace(n_rec), "cbm_scatter: workspace too small");
+    ARG(ws && ws_bytes >= cbm_scatter_workspace(n_rec), "cbm_scatter: workspace too small");
     int rc;
     cbm_columns c = *cols;
+    int32_t* ncig = (int32_t*)ws;
+    int32_t* lseq = ncig + n_rec;
+    void* scan_ws = (uint8_t*)ws + ((8 * n_rec + 255) / 256) * 256;
     if (n_rec > 0) {
         ARG(data && offsets, "cbm_scatter: null pointer");
-        unsigned g = (unsigned)((n_rec + SC_NT - 1) / SC_NT);
-        hipLaunchKernelGGL(rec_fields, dim3(g), dim3(SC_NT), 0, st, data, offsets, n_rec, c);
+        unsigned g = (unsigned)((n_rec + FIELDS_NT - 1) / FIELDS_NT);
+        hipLaunchKernelGGL(rec_fields, dim3(g), dim3(FIELDS_THREADS), 0, st, data, offsets, n_rec, c, ncig, lseq);
         if ((rc = check_launch("rec_fields"))) return rc;
     }
     if (c.cigar_off &&
-        (rc = exclusive_scan(RecNcig{data, offsets}, StoreI64{c.cigar_off}, (const int64_t*)nullptr, n_rec, ws,
+        (rc = exclusive_scan(RecNcig{ncig}, StoreI64{c.cigar_off}, (const int64_t*)nullptr, n_rec, scan_ws,
                              (int64_t*)nullptr, st)))
         return rc;
     if (c.seq_off &&
-        (rc = exclusive_scan(RecSeqB{data, offsets}, StoreI64{c.seq_off}, (const int64_t*)nullptr, n_rec, ws,
+        (rc = exclusive_scan(RecSeqB{lseq}, StoreI64{c.seq_off}, (const int64_t*)nullptr, n_rec, scan_ws,
                              (int64_t*)nullptr, st)))
         return rc;
     if (c.qual_off &&
-        (rc = exclusive_scan(RecQual{data, offsets}, StoreI64{c.qual_off}, (const int64_t*)nullptr, n_rec, ws,
+        (rc = exclusive_scan(RecQual{lseq}, StoreI64{c.qual_off}, (const int64_t*)nullptr, n_rec, scan_ws,
                              (int64_t*)nullptr, st)))
         return rc;
     if (n_rec > 0 && (c.cigar || c.seq || c.qual)) {
-        unsigned g = (unsigned)((n_rec + SC_NT - 1) / SC_NT);
-        hipLaunchKernelGGL(rec_copy_var, dim3(g), dim3(SC_NT), 0, st, data, offsets, n_rec, c);
+        unsigned g = (unsigned)((n_rec + FIELDS_NT - 1) / FIELDS_NT);
+        hipLaunchKernelGGL(rec_copy_var, dim3(g), dim3(FIELDS_THREADS), 0, st, data, offsets, n_rec, c);
         if ((rc = check_launch("rec_copy_var"))) return rc;
     }
     return CBM_OK;

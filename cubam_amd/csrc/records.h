@@ -297,40 +297,41 @@ __global__ void recs_bases(SpanTab t, RecWs w, int64_t* span_rec) {
 // ------------------------------------------------------------------ kernel 4: scatter
 constexpr int SC_NT = 256;
 
-// decode_fixed (_kernels.py:292-319) and find_tag (_kernels.py:334-388), one thread per record
-__global__ void __launch_bounds__(SC_NT) rec_fields(const uint8_t* b, const int64_t* off, int64_t n,
-                                                    cbm_columns c) {
-    int64_t i = (int64_t)blockIdx.x * SC_NT + threadIdx.x;
-    if (i >= n) return;
-    int64_t o = off[i];
-    int32_t p = (int32_t)ld_u32(b, o + 8);
-    uint32_t lseq = ld_u32(b, o + 20);
-    if (c.ref) c.ref[i] = (int32_t)ld_u32(b, o + 4);
-    if (c.pos) c.pos[i] = p;
-    if (c.mapq) c.mapq[i] = b[o + 13];
-    if (c.flag) c.flag[i] = (uint16_t)ld_u16(b, o + 18);
-    if (c.nref) c.nref[i] = (int32_t)ld_u32(b, o + 24);
-    if (c.npos) c.npos[i] = (int32_t)ld_u32(b, o + 28);
-    if (c.tlen) c.tlen[i] = (int32_t)ld_u32(b, o + 32);
+// decode_fixed (_kernels.py:292-319) and find_tag (_kernels.py:334-388) of the record at p + o;
+// gbase + (a position in p) is that position in the data (for the tag value offsets). Also
+// writes n_cigar_op and l_seq of the record to ncig[i] and lseq[i] for the CSR offsets.
+__device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_t gbase, int64_t i,
+                                             const cbm_columns& c, int32_t* ncig_out, int32_t* lseq_out) {
+    int32_t pos = (int32_t)ld_u32(p, o + 8);
+    uint32_t lseq = ld_u32(p, o + 20);
+    if (c.ref) c.ref[i] = (int32_t)ld_u32(p, o + 4);
+    if (c.pos) c.pos[i] = pos;
+    if (c.mapq) c.mapq[i] = p[o + 13];
+    if (c.flag) c.flag[i] = (uint16_t)ld_u16(p, o + 18);
+    if (c.nref) c.nref[i] = (int32_t)ld_u32(p, o + 24);
+    if (c.npos) c.npos[i] = (int32_t)ld_u32(p, o + 28);
+    if (c.tlen) c.tlen[i] = (int32_t)ld_u32(p, o + 32);
     if (c.l_seq) c.l_seq[i] = (int32_t)lseq;
-    uint32_t lrn = b[o + 12], ncig = ld_u16(b, o + 16);
+    uint32_t lrn = p[o + 12], ncig = ld_u16(p, o + 16);
+    ncig_out[i] = (int32_t)ncig;
+    lseq_out[i] = (int32_t)lseq;
     int64_t cg = o + 36 + lrn, rlen = 0;
     uint32_t cl = 0, cr = 0;
     for (uint32_t k = 0; k < ncig; k++) {
-        uint32_t v = ld_u32(b, cg + 4LL * k), op = v & 15, len = v >> 4;
+        uint32_t v = ld_u32(p, cg + 4LL * k), op = v & 15, len = v >> 4;
         if (ref_consuming(op)) rlen += len;
         if (op == 4) {
             if (k == 0) cl = len;
             else cr = len;
         }
     }
-    if (c.end) c.end[i] = p + (int32_t)rlen;
+    if (c.end) c.end[i] = pos + (int32_t)rlen;
     if (c.clip_left) c.clip_left[i] = (uint16_t)(cl > 65535 ? 65535 : cl);
     if (c.clip_right) c.clip_right[i] = (uint16_t)(cr > 65535 ? 65535 : cr);
     int nt = c.n_tags;
     if (nt <= 0) return;
     // aux fields, once for every tag asked for
-    int64_t rec_end = o + 4 + (int64_t)ld_u32(b, o);
+    int64_t rec_end = o + 4 + (int64_t)ld_u32(p, o);
     int64_t a = cg + 4LL * ncig + (lseq + 1) / 2 + lseq;
     uint32_t found = 0;
     bool bad = false;
@@ -339,7 +340,7 @@ __global__ void __launch_bounds__(SC_NT) rec_fields(const uint8_t* b, const int6
     int32_t vl[4] = {0, 0, 0, 0};
     uint32_t want = (1u << nt) - 1;
     while (a + 3 <= rec_end && found != want) {
-        uint32_t ty = b[a + 2];
+        uint32_t ty = p[a + 2];
         int64_t v = a + 3, nx;
         switch (ty) {
             case 'A': case 'c': case 'C': nx = v + 1; break;
@@ -347,13 +348,13 @@ __global__ void __launch_bounds__(SC_NT) rec_fields(const uint8_t* b, const int6
             case 'i': case 'I': case 'f': nx = v + 4; break;
             case 'Z': case 'H':
                 nx = v;
-                while (nx < rec_end && b[nx] != 0) nx++;
+                while (nx < rec_end && p[nx] != 0) nx++;
                 nx++;
                 break;
             case 'B': {
                 if (v + 5 > rec_end) { nx = -1; break; }
-                uint32_t sub = b[v];
-                int64_t cnt = (int64_t)ld_u32(b, v + 1);
+                uint32_t sub = p[v];
+                int64_t cnt = (int64_t)ld_u32(p, v + 1);
                 int sz = (sub == 'c' || sub == 'C') ? 1 : (sub == 's' || sub == 'S') ? 2 : 4;
                 nx = v + 5 + cnt * sz;
                 break;
@@ -361,21 +362,21 @@ __global__ void __launch_bounds__(SC_NT) rec_fields(const uint8_t* b, const int6
             default: nx = -1;
         }
         if (nx < 0 || nx > rec_end) { bad = true; break; }
-        uint32_t t0 = b[a], t1 = b[a + 1];
+        uint32_t t0 = p[a], t1 = p[a + 1];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             if (t < nt && !(found & (1u << t)) && t0 == (uint8_t)c.tag_names[2 * t] &&
                 t1 == (uint8_t)c.tag_names[2 * t + 1]) {
                 found |= 1u << t;
                 ty_out[t] = (uint8_t)ty;
-                vo[t] = v;
+                vo[t] = gbase + v;
                 switch (ty) {
-                    case 'c': iv[t] = (int8_t)b[v]; break;
-                    case 'C': case 'A': iv[t] = b[v]; break;
-                    case 's': iv[t] = (int16_t)ld_u16(b, v); break;
-                    case 'S': iv[t] = ld_u16(b, v); break;
-                    case 'i': iv[t] = (int32_t)ld_u32(b, v); break;
-                    case 'I': iv[t] = ld_u32(b, v); break;
+                    case 'c': iv[t] = (int8_t)p[v]; break;
+                    case 'C': case 'A': iv[t] = p[v]; break;
+                    case 's': iv[t] = (int16_t)ld_u16(p, v); break;
+                    case 'S': iv[t] = ld_u16(p, v); break;
+                    case 'i': iv[t] = (int32_t)ld_u32(p, v); break;
+                    case 'I': iv[t] = ld_u32(p, v); break;
                     case 'Z': case 'H': vl[t] = (int32_t)(nx - 1 - v); break;
                     case 'B': vl[t] = (int32_t)(nx - v); break;
                     default: break;
@@ -396,63 +397,207 @@ __global__ void __launch_bounds__(SC_NT) rec_fields(const uint8_t* b, const int6
     }
 }
 
-// CSR columns: a wavefront copies the CIGAR words, packed bases and qualities of 64
-// consecutive records, one record at a time with all lanes, so writes are contiguous.
-__global__ void __launch_bounds__(SC_NT) rec_copy_var(const uint8_t* b, const int64_t* off, int64_t n,
-                                                      cbm_columns c) {
-    int lane = threadIdx.x % CBM_WAVE;
-    int64_t g = ((int64_t)blockIdx.x * SC_NT + threadIdx.x) / CBM_WAVE;
-    int64_t r0 = g * CBM_WAVE;
-    if (r0 >= n) return;
-    int64_t my = r0 + lane;
-    int64_t src = 0, co = 0, so = 0, qo = 0;
-    int32_t ncig = 0, lseq = 0;
-    if (my < n) {
-        int64_t o = off[my];
-        uint32_t lrn = b[o + 12];
-        ncig = (int32_t)ld_u16(b, o + 16);
-        lseq = (int32_t)ld_u32(b, o + 20);
-        src = o + 36 + lrn;
-        co = c.cigar_off ? c.cigar_off[my] : 0;
-        so = c.seq_off ? c.seq_off[my] : 0;
-        qo = c.qual_off ? c.qual_off[my] : 0;
-    }
-    int m = (int)(n - r0 < CBM_WAVE ? n - r0 : CBM_WAVE);
-    for (int r = 0; r < m; r++) {
-        int64_t s = __shfl(src, r, CBM_WAVE);
-        int32_t nc = __shfl(ncig, r, CBM_WAVE), ls = __shfl(lseq, r, CBM_WAVE);
-        if (c.cigar) {
-            int64_t d = __shfl(co, r, CBM_WAVE);
-            for (int k = lane; k < nc; k += CBM_WAVE) c.cigar[d + k] = ld_u32(b, s + 4LL * k);
+// One workgroup per FIELDS_NT consecutive records. Records found by the chain are back to back,
+// so the workgroup's records are one byte range: it is staged in LDS with coalesced 16-byte
+// loads, and each thread decodes its record from there (one thread per record reading the
+// data in place would touch 64 cache lines per load instruction of a wavefront). A record
+// outside the staged range (offsets that are not consecutive records, or a range larger than
+// the stage) is decoded from global memory.
+constexpr int SC_STAGE = 40960;
+constexpr int FIELDS_NT = 128;  // records per workgroup: 128 GEX-like records (~250 B) fit the stage
+constexpr int FIELDS_THREADS = 256;  // threads per workgroup (all stage and copy; FIELDS_NT decode)
+
+// Stage data bytes [a0, a0 + nb) (a0 16-byte aligned in memory, nb <= SC_STAGE) into st with
+// 16-byte loads, four in flight per thread.
+__device__ __forceinline__ void stage_records(uint8_t* st, const uint8_t* b, int64_t a0, int64_t nb) {
+    constexpr int SB = 4;
+    for (int64_t k0 = (int64_t)threadIdx.x * 16; k0 < nb; k0 += (int64_t)FIELDS_THREADS * 16 * SB) {
+        uint4 t[SB];
+#pragma unroll
+        for (int u = 0; u < SB; u++) {
+            const int64_t k = k0 + (int64_t)u * FIELDS_THREADS * 16;
+            if (k < nb) t[u] = *(const uint4*)(b + a0 + k);
         }
-        int64_t sq = s + 4LL * nc;
-        int32_t nb = (ls + 1) / 2;
-        if (c.seq) {
-            int64_t d = __shfl(so, r, CBM_WAVE);
-            for (int k = lane; k < nb; k += CBM_WAVE) c.seq[d + k] = b[sq + k];
-        }
-        if (c.qual) {
-            int64_t d = __shfl(qo, r, CBM_WAVE);
-            for (int k = lane; k < ls; k += CBM_WAVE) c.qual[d + k] = b[sq + nb + k];
+#pragma unroll
+        for (int u = 0; u < SB; u++) {
+            const int64_t k = k0 + (int64_t)u * FIELDS_THREADS * 16;
+            if (k < nb) *(uint4*)(st + k) = t[u];
         }
     }
 }
 
-// per-record sizes for the CSR offsets
+__global__ void __launch_bounds__(FIELDS_THREADS) rec_fields(const uint8_t* b, const int64_t* off, int64_t n,
+                                                    cbm_columns c, int32_t* ncig, int32_t* lseq) {
+    __shared__ __align__(16) uint8_t st[SC_STAGE + 16];
+    __shared__ int64_t range[2];
+    const int64_t i0 = (int64_t)blockIdx.x * FIELDS_NT;
+    const int64_t i1 = n < i0 + FIELDS_NT ? n : i0 + FIELDS_NT;
+    const int64_t i = i0 + threadIdx.x;
+    if (threadIdx.x == 0) {
+        const int64_t lo = off[i0], last = off[i1 - 1];
+        const int64_t hi = last + 4 + (int64_t)ld_u32(b, last);
+        // start at a 16-byte aligned address (never below the allocation: device allocations are
+        // 256-byte aligned, and the data pointer lies inside one)
+        const int64_t a0 = lo - (int64_t)(((uintptr_t)(b + lo)) & 15);
+        range[0] = a0;
+        range[1] = (last >= lo && hi - a0 <= SC_STAGE) ? hi : a0;
+    }
+    __syncthreads();
+    const int64_t a0 = range[0], hi = range[1];
+    stage_records(st, b, a0, hi - a0);
+    __syncthreads();
+    if (threadIdx.x >= FIELDS_NT || i >= n) return;
+    const int64_t o = off[i];
+    bool staged = o >= a0 && o + 36 <= hi;
+    if (staged) staged = o + 4 + (int64_t)ld_u32(st, o - a0) <= hi;
+    if (staged) parse_record(st, o - a0, a0, i, c, ncig, lseq);
+    else parse_record(b, o, 0, i, c, ncig, lseq);
+}
+
+// CSR columns. A workgroup stages its FIELDS_NT consecutive records in LDS (as rec_fields),
+// then writes each column's output range, which is contiguous across its records: threads
+// take consecutive pieces of it, find the record of a piece's first element by a binary
+// search over the records' CSR offsets (in LDS) and read the elements from the stage. Writes
+// are coalesced whatever the records' lengths. Records outside the stage are read in place.
+__device__ __forceinline__ int csr_record(const int64_t* sh_off, int m, int64_t j) {
+    // the record r with sh_off[r] <= j < sh_off[r + 1]
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sh_off[mid] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// CIGAR words: one thread per word
+__device__ __forceinline__ void csr_fill_words(uint32_t* out, const int64_t* ooff, int64_t i0, int m,
+                                               const int64_t* sh_off, const int64_t* sh_src, const uint8_t* st,
+                                               int64_t a0, int64_t hi, const uint8_t* b) {
+    const int64_t base = ooff[i0], total = ooff[i0 + m] - base;
+    for (int64_t j = threadIdx.x; j < total; j += FIELDS_THREADS) {
+        const int r = csr_record(sh_off, m, j);
+        const int64_t src = sh_src[r] + 4 * (j - sh_off[r]);
+        out[base + j] = (src >= a0 && src + 4 <= hi) ? ld_u32(st, src - a0) : ld_u32(b, src);
+    }
+}
+
+// Bases or qualities: each thread fills one 16-byte aligned piece of the output (one search,
+// then a walk over the records it spans), stored with one 16-byte store when the piece lies
+// wholly in this workgroup's range.
+__device__ __forceinline__ void csr_fill_bytes(uint8_t* out, const int64_t* ooff, int64_t i0, int m,
+                                               const int64_t* sh_off, const int64_t* sh_src, const uint8_t* st,
+                                               int64_t a0, int64_t hi, const uint8_t* b) {
+    const int64_t base = ooff[i0], end = ooff[i0 + m];
+    if (end <= base) return;
+    const uintptr_t ob = (uintptr_t)out;
+    // pieces: [p, p + 16) with out + p 16-byte aligned
+    const int64_t p0 = base - (int64_t)((ob + base) & 15);
+    for (int64_t p = p0 + 16 * (int64_t)threadIdx.x; p < end; p += 16 * (int64_t)FIELDS_THREADS) {
+        const int64_t lo = p < base ? base : p, up = p + 16 < end ? p + 16 : end;
+        int r = csr_record(sh_off, m, lo - base);
+        int64_t nxt = r + 1 < m ? sh_off[r + 1] + base : end;
+        int64_t src = sh_src[r] + (lo - base - sh_off[r]);
+        uint8_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int64_t j = p + k;
+            uint8_t x = 0;
+            if (j >= lo && j < up) {
+                while (j >= nxt) {  // next record (empty ones skipped)
+                    r++;
+                    src = sh_src[r];
+                    nxt = r + 1 < m ? sh_off[r + 1] + base : end;
+                }
+                x = (src >= a0 && src < hi) ? st[src - a0] : b[src];
+                src++;
+            }
+            v[k] = x;
+        }
+        if (lo == p && up == p + 16) {
+            uint4 w;
+            w.x = v[0] | (v[1] << 8) | (v[2] << 16) | ((uint32_t)v[3] << 24);
+            w.y = v[4] | (v[5] << 8) | (v[6] << 16) | ((uint32_t)v[7] << 24);
+            w.z = v[8] | (v[9] << 8) | (v[10] << 16) | ((uint32_t)v[11] << 24);
+            w.w = v[12] | (v[13] << 8) | (v[14] << 16) | ((uint32_t)v[15] << 24);
+            *(uint4*)(out + p) = w;
+        } else {
+            for (int64_t j = lo; j < up; j++) out[j] = v[j - p];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(FIELDS_THREADS) rec_copy_var(const uint8_t* b, const int64_t* off, int64_t n,
+                                                          cbm_columns c) {
+    __shared__ __align__(16) uint8_t st[SC_STAGE + 16];
+    __shared__ int64_t sh_off[FIELDS_NT + 1], sh_src[FIELDS_NT];
+    __shared__ int64_t range[2];
+    const int64_t i0 = (int64_t)blockIdx.x * FIELDS_NT;
+    if (i0 >= n) return;
+    const int m = (int)(n - i0 < FIELDS_NT ? n - i0 : FIELDS_NT);
+    if (threadIdx.x == 0) {
+        const int64_t lo = off[i0], last = off[i0 + m - 1];
+        const int64_t hi = last + 4 + (int64_t)ld_u32(b, last);
+        const int64_t a0 = lo - (int64_t)(((uintptr_t)(b + lo)) & 15);
+        range[0] = a0;
+        range[1] = (last >= lo && hi - a0 <= SC_STAGE) ? hi : a0;
+    }
+    __syncthreads();
+    const int64_t a0 = range[0], hi = range[1];
+    stage_records(st, b, a0, hi - a0);
+    __syncthreads();
+    // where each record's CIGAR, bases and qualities start (data offsets)
+    int64_t cig = 0, sq = 0, ql = 0;
+    if (threadIdx.x < m) {
+        const int64_t o = off[i0 + threadIdx.x];
+        const bool staged = o >= a0 && o + 36 <= hi;
+        const uint8_t* p = staged ? st : b;
+        const int64_t r = staged ? o - a0 : o;
+        const uint32_t lrn = p[r + 12], ncig = ld_u16(p, r + 16), lseq = ld_u32(p, r + 20);
+        cig = o + 36 + lrn;
+        sq = cig + 4LL * ncig;
+        ql = sq + (lseq + 1) / 2;
+    }
+    if (c.cigar && c.cigar_off) {
+        if (threadIdx.x < m) {
+            sh_off[threadIdx.x] = c.cigar_off[i0 + threadIdx.x] - c.cigar_off[i0];
+            sh_src[threadIdx.x] = cig;
+        }
+        __syncthreads();
+        csr_fill_words(c.cigar, c.cigar_off, i0, m, sh_off, sh_src, st, a0, hi, b);
+        __syncthreads();
+    }
+    if (c.seq && c.seq_off) {
+        if (threadIdx.x < m) {
+            sh_off[threadIdx.x] = c.seq_off[i0 + threadIdx.x] - c.seq_off[i0];
+            sh_src[threadIdx.x] = sq;
+        }
+        __syncthreads();
+        csr_fill_bytes(c.seq, c.seq_off, i0, m, sh_off, sh_src, st, a0, hi, b);
+        __syncthreads();
+    }
+    if (c.qual && c.qual_off) {
+        if (threadIdx.x < m) {
+            sh_off[threadIdx.x] = c.qual_off[i0 + threadIdx.x] - c.qual_off[i0];
+            sh_src[threadIdx.x] = ql;
+        }
+        __syncthreads();
+        csr_fill_bytes(c.qual, c.qual_off, i0, m, sh_off, sh_src, st, a0, hi, b);
+    }
+}
+
+// per-record sizes for the CSR offsets, from the arrays rec_fields writes
 struct RecNcig {
-    const uint8_t* b;
-    const int64_t* off;
-    __device__ int64_t operator()(int64_t i) const { return ld_u16(b, off[i] + 16); }
+    const int32_t* ncig;
+    __device__ int64_t operator()(int64_t i) const { return ncig[i]; }
 };
 struct RecSeqB {
-    const uint8_t* b;
-    const int64_t* off;
-    __device__ int64_t operator()(int64_t i) const { return (ld_u32(b, off[i] + 20) + 1) / 2; }
+    const int32_t* lseq;
+    __device__ int64_t operator()(int64_t i) const { return ((int64_t)(uint32_t)lseq[i] + 1) / 2; }
 };
 struct RecQual {
-    const uint8_t* b;
-    const int64_t* off;
-    __device__ int64_t operator()(int64_t i) const { return ld_u32(b, off[i] + 20); }
+    const int32_t* lseq;
+    __device__ int64_t operator()(int64_t i) const { return (int64_t)(uint32_t)lseq[i]; }
 };
 
 }  // namespace cbm
