@@ -14,6 +14,8 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libcubam_amd.so"
 
 PAD = 64  # CBM_PAD: bytes readable past the end of every input buffer
+DEFLATE_SLOT = 65536 + 256  # CBM_DEFLATE_SLOT
+DEFLATE_MAX_IN = 65280  # CBM_DEFLATE_MAX_IN
 
 # result slots and codes (mirrors of the header's enums)
 SCAN_NBLOCKS, SCAN_USED, SCAN_STATUS, SCAN_TOTAL_OUT, SCAN_NCAND = 0, 1, 2, 3, 4
@@ -83,6 +85,9 @@ SYMBOLS = (
     "cbm_record_offsets_spans",
     "cbm_scatter_workspace",
     "cbm_scatter",
+    "cbm_deflate_workspace",
+    "cbm_deflate",
+    "cbm_copy_spans",
     "cbm_last_error",
     "cbm_version",
 )
@@ -131,6 +136,12 @@ def load():
     L.cbm_scatter_workspace.argtypes = [i64]
     L.cbm_scatter.restype = ctypes.c_int
     L.cbm_scatter.argtypes = [vp, vp, i64, ctypes.POINTER(Columns), vp, i64, vp]
+    L.cbm_deflate_workspace.restype = i64
+    L.cbm_deflate_workspace.argtypes = [i64]
+    L.cbm_deflate.restype = ctypes.c_int
+    L.cbm_deflate.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp, i64, vp]
+    L.cbm_copy_spans.restype = ctypes.c_int
+    L.cbm_copy_spans.argtypes = [vp, vp, vp, vp, i64, vp, vp]
     L.cbm_last_error.restype = ctypes.c_int
     L.cbm_last_error.argtypes = [ctypes.c_char_p, i64]
     L.cbm_version.restype = ctypes.c_char_p

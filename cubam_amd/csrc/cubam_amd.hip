@@ -9,6 +9,7 @@
 
 #include "bgzf_scan.h"
 #include "common.h"
+#include "deflate.h"
 #include "inflate.h"
 #include "inflate_wave.h"
 #include "records.h"
@@ -507,6 +508,42 @@ int cbm_record_offsets_spans(const uint8_t* data, int64_t n_seg, const void* ws,
     hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, (int64_t)0, (int64_t)0, w,
                        offsets, n_rec);
     return check_launch("rec_emit");
+}
+
+// ------------------------------------------------------------------ kernel 5: BGZF compression
+static int64_t deflate_groups(int64_t n_blocks) {
+    int64_t g = (n_blocks + DF_WAVES - 1) / DF_WAVES;
+    int64_t cap = (int64_t)device_cus() * 2;  // the LDS of a CU holds two workgroups
+    return g < cap ? (g < 1 ? 1 : g) : cap;
+}
+
+int64_t cbm_deflate_workspace(int64_t n_blocks) {
+    if (n_blocks < 0) return -1;
+    return deflate_groups(n_blocks) * DF_WAVES * (int64_t)65536 * 4;
+}
+
+int cbm_deflate(const uint8_t* src, const int64_t* in_off, const int32_t* in_len, int64_t n_blocks, uint8_t* slots,
+                int32_t* bsize, uint32_t* crc, void* ws, int64_t ws_bytes, void* stream) {
+    ARG(n_blocks >= 0, "cbm_deflate: bad arguments");
+    if (n_blocks == 0) return CBM_OK;
+    ARG(src && in_off && in_len && slots && bsize && ws, "cbm_deflate: null pointer");
+    ARG(ws_bytes >= cbm_deflate_workspace(n_blocks), "cbm_deflate: workspace too small");
+    const int64_t g = deflate_groups(n_blocks);
+    hipLaunchKernelGGL(deflate_blocks, dim3((unsigned)g), dim3(DF_NT), 0, (hipStream_t)stream, src, in_off, in_len,
+                       n_blocks, slots, bsize, crc, (uint32_t*)ws, (int64_t)65536);
+    return check_launch("deflate_blocks");
+}
+
+int cbm_copy_spans(const uint8_t* src, const int64_t* src_off, const int64_t* len, const int64_t* dst_off, int64_t n,
+                   uint8_t* dst, void* stream) {
+    ARG(n >= 0, "cbm_copy_spans: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(src && src_off && len && dst_off && dst, "cbm_copy_spans: null pointer");
+    int64_t g = (n + 3) / 4;
+    if (g > (int64_t)device_cus() * 16) g = (int64_t)device_cus() * 16;
+    hipLaunchKernelGGL(copy_spans, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, src, src_off, len, dst_off,
+                       n, dst);
+    return check_launch("copy_spans");
 }
 
 // ------------------------------------------------------------------ kernel 4

@@ -236,6 +236,34 @@ int64_t cbm_scatter_workspace(int64_t n_rec);
 int cbm_scatter(const uint8_t* data_d, const int64_t* offsets_d, int64_t n_rec, const cbm_columns* cols, void* ws_d,
                 int64_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------ BGZF compression (kernel 5) */
+
+/* Output bytes reserved per block by cbm_deflate (a BGZF block is at most 65,536 bytes). */
+#define CBM_DEFLATE_SLOT (65536 + 256)
+/* Payload bytes a block may hold (htslib's BGZF_BLOCK_SIZE cut). */
+#define CBM_DEFLATE_MAX_IN 65280
+
+/* Bytes of workspace cbm_deflate needs for n_blocks blocks. */
+int64_t cbm_deflate_workspace(int64_t n_blocks);
+
+/*
+ * Compress n_blocks payloads src_d[in_off[i] : + in_len[i]) (each at most
+ * CBM_DEFLATE_MAX_IN bytes) into whole BGZF blocks (gzip header with the BC
+ * extra field, raw DEFLATE, CRC32 and ISIZE trailer), block i at
+ * slots_d + i * CBM_DEFLATE_SLOT; bsize_d[i] receives its size in bytes and
+ * crc_d[i] (may be NULL) the payload's CRC32. Greedy level-1 matching and
+ * dynamic Huffman codes; a block that does not shrink is stored. Replaces
+ * _bgzf_compress (src/cubam/bgzf.py:380-422: nvCOMP Deflate, crc32_spans and
+ * the bgzf_frame kernel of src/cubam/_kernels.py:161-191).
+ */
+int cbm_deflate(const uint8_t* src_d, const int64_t* in_off_d, const int32_t* in_len_d, int64_t n_blocks,
+                uint8_t* slots_d, int32_t* bsize_d, uint32_t* crc_d, void* ws_d, int64_t ws_bytes, void* stream);
+
+/* Copy n spans src_d[src_off[i] : + len[i]) to dst_d + dst_off[i], one wavefront per span
+   (record gathers and block packing). Replaces copy_spans (src/cubam/_kernels.py:34-43, 644-659). */
+int cbm_copy_spans(const uint8_t* src_d, const int64_t* src_off_d, const int64_t* len_d, const int64_t* dst_off_d,
+                   int64_t n, uint8_t* dst_d, void* stream);
+
 /* ------------------------------------------------------------------ misc */
 
 /* Copy the last error message of this thread into buf (NUL-terminated); returns its length. */
