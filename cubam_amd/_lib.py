@@ -85,6 +85,12 @@ SYMBOLS = (
     "cbm_record_offsets_spans",
     "cbm_scatter_workspace",
     "cbm_scatter",
+    "cbm_name_hash",
+    "cbm_hash_spans",
+    "cbm_pack_acgt",
+    "cbm_aligned_length",
+    "cbm_blocks",
+    "cbm_base_at",
     "cbm_deflate_workspace",
     "cbm_deflate",
     "cbm_copy_spans",
@@ -136,6 +142,12 @@ def load():
     L.cbm_scatter_workspace.argtypes = [i64]
     L.cbm_scatter.restype = ctypes.c_int
     L.cbm_scatter.argtypes = [vp, vp, i64, ctypes.POINTER(Columns), vp, i64, vp]
+    for f, at in ((L.cbm_name_hash, [vp, vp, i64, vp, vp]), (L.cbm_hash_spans, [vp, vp, vp, i64, vp, vp]),
+                  (L.cbm_pack_acgt, [vp, vp, vp, i64, vp, vp, vp]), (L.cbm_aligned_length, [vp, vp, i64, vp, vp]),
+                  (L.cbm_blocks, [vp, vp, i64, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp]),
+                  (L.cbm_base_at, [vp, vp, vp, vp, i64, vp, vp])):
+        f.restype = ctypes.c_int
+        f.argtypes = at
     L.cbm_deflate_workspace.restype = i64
     L.cbm_deflate_workspace.argtypes = [i64]
     L.cbm_deflate.restype = ctypes.c_int

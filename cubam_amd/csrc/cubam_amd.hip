@@ -12,6 +12,7 @@
 #include "deflate.h"
 #include "inflate.h"
 #include "inflate_wave.h"
+#include "ondemand.h"
 #include "records.h"
 
 namespace cbm {
@@ -508,6 +509,64 @@ int cbm_record_offsets_spans(const uint8_t* data, int64_t n_seg, const void* ws,
     hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, (int64_t)0, (int64_t)0, w,
                        offsets, n_rec);
     return check_launch("rec_emit");
+}
+
+// ------------------------------------------------------------------ on-demand record kernels
+static unsigned od_grid(int64_t n) { return (unsigned)((n + OD_NT - 1) / OD_NT); }
+
+int cbm_name_hash(const uint8_t* data, const int64_t* off, int64_t n, uint64_t* out, void* stream) {
+    ARG(n >= 0, "cbm_name_hash: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(data && off && out, "cbm_name_hash: null pointer");
+    hipLaunchKernelGGL(od_name_hash, dim3(od_grid(n)), dim3(OD_NT), 0, (hipStream_t)stream, data, off, n, out);
+    return check_launch("od_name_hash");
+}
+
+int cbm_hash_spans(const uint8_t* data, const int64_t* voff, const int32_t* vlen, int64_t n, uint64_t* out,
+                   void* stream) {
+    ARG(n >= 0, "cbm_hash_spans: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(data && voff && vlen && out, "cbm_hash_spans: null pointer");
+    hipLaunchKernelGGL(od_hash_spans, dim3(od_grid(n)), dim3(OD_NT), 0, (hipStream_t)stream, data, voff, vlen, n, out);
+    return check_launch("od_hash_spans");
+}
+
+int cbm_pack_acgt(const uint8_t* data, const int64_t* voff, const int32_t* vlen, int64_t n, uint64_t* out,
+                  uint8_t* ok, void* stream) {
+    ARG(n >= 0, "cbm_pack_acgt: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(data && voff && vlen && out && ok, "cbm_pack_acgt: null pointer");
+    hipLaunchKernelGGL(od_pack_acgt, dim3(od_grid(n)), dim3(OD_NT), 0, (hipStream_t)stream, data, voff, vlen, n, out,
+                       ok);
+    return check_launch("od_pack_acgt");
+}
+
+int cbm_aligned_length(const uint8_t* data, const int64_t* off, int64_t n, int32_t* out, void* stream) {
+    ARG(n >= 0, "cbm_aligned_length: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(data && off && out, "cbm_aligned_length: null pointer");
+    hipLaunchKernelGGL(od_aligned_len, dim3(od_grid(n)), dim3(OD_NT), 0, (hipStream_t)stream, data, off, n, out);
+    return check_launch("od_aligned_len");
+}
+
+int cbm_blocks(const uint8_t* data, const int64_t* off, int64_t n, int split_deletions, int fill, const int64_t* first,
+               int32_t* cnt, int64_t* rec, int32_t* start, int32_t* end, void* stream) {
+    ARG(n >= 0, "cbm_blocks: bad arguments");
+    if (n == 0) return CBM_OK;
+    ARG(data && off && cnt && (!fill || (first && rec && start && end)), "cbm_blocks: null pointer");
+    hipLaunchKernelGGL(od_blocks, dim3(od_grid(n)), dim3(OD_NT), 0, (hipStream_t)stream, data, off, n, split_deletions,
+                       fill, first, cnt, rec, start, end);
+    return check_launch("od_blocks");
+}
+
+int cbm_base_at(const uint8_t* data, const int64_t* off, const int64_t* rec, const int32_t* refpos, int64_t m,
+                int8_t* out, void* stream) {
+    ARG(m >= 0, "cbm_base_at: bad arguments");
+    if (m == 0) return CBM_OK;
+    ARG(data && off && rec && refpos && out, "cbm_base_at: null pointer");
+    hipLaunchKernelGGL(od_base_at, dim3(od_grid(m)), dim3(OD_NT), 0, (hipStream_t)stream, data, off, rec, refpos, m,
+                       out);
+    return check_launch("od_base_at");
 }
 
 // ------------------------------------------------------------------ kernel 5: BGZF compression

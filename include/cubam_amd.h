@@ -236,6 +236,42 @@ int64_t cbm_scatter_workspace(int64_t n_rec);
 int cbm_scatter(const uint8_t* data_d, const int64_t* offsets_d, int64_t n_rec, const cbm_columns* cols, void* ws_d,
                 int64_t ws_bytes, void* stream);
 
+/* ------------------------------------------------------------------ on-demand record kernels */
+/* One thread per record, reading records in place at data_d + off_d[i] (batch offsets). */
+
+/* FNV-1a 64 (basis 1469598103934665603, as the reference) of each read name. Replaces the
+   name_hash kernel of RecordBatch.name_hash (src/cubam/bam.py:322-330, _kernels.py:322-330). */
+int cbm_name_hash(const uint8_t* data_d, const int64_t* off_d, int64_t n, uint64_t* out_d, void* stream);
+
+/* FNV-1a 64 of spans data_d[voff[i] : + vlen[i]), 0 where vlen[i] < 0. Replaces hash_spans of
+   Tag.hash (src/cubam/bam.py:221-230, _kernels.py:533-541). */
+int cbm_hash_spans(const uint8_t* data_d, const int64_t* voff_d, const int32_t* vlen_d, int64_t n, uint64_t* out_d,
+                   void* stream);
+
+/* Values [ACGT]{1,24}(-[0-9]{1,3})? packed into uint64 codes (2 bits a base, base count at bit
+   48, suffix + 1 at bit 53); ok_d[i] = 0 and code 0 where a value does not fit. Replaces
+   pack_acgt of Tag.packed (src/cubam/bam.py:232-248, _kernels.py:543-571). */
+int cbm_pack_acgt(const uint8_t* data_d, const int64_t* voff_d, const int32_t* vlen_d, int64_t n, uint64_t* out_d,
+                  uint8_t* ok_d, void* stream);
+
+/* Aligned bases (CIGAR M, =, X) of each read. Replaces aligned_len of
+   RecordBatch.aligned_length (src/cubam/bam.py:341-348, _kernels.py:390-403). */
+int cbm_aligned_length(const uint8_t* data_d, const int64_t* off_d, int64_t n, int32_t* out_d, void* stream);
+
+/* Aligned blocks of each read (runs of M/=/X, and D unless split_deletions, between N skips):
+   with fill = 0, cnt_d[i] = blocks of read i; with fill = 1, blocks written from first_d[i] as
+   (rec, start, end). Replaces blocks of RecordBatch.blocks (src/cubam/bam.py:350-373,
+   _kernels.py:405-437). */
+int cbm_blocks(const uint8_t* data_d, const int64_t* off_d, int64_t n, int split_deletions, int fill,
+               const int64_t* first_d, int32_t* cnt_d, int64_t* rec_d, int32_t* start_d, int32_t* end_d,
+               void* stream);
+
+/* Base of read rec_d[j] at reference position refpos_d[j]: 0..3 (A C G T), 4 other, -1 deletion,
+   -2 reference skip, -3 not covered. Replaces base_at of RecordBatch.base_at
+   (src/cubam/bam.py:375-389, _kernels.py:498-529). */
+int cbm_base_at(const uint8_t* data_d, const int64_t* off_d, const int64_t* rec_d, const int32_t* refpos_d, int64_t m,
+                int8_t* out_d, void* stream);
+
 /* ------------------------------------------------------------------ BGZF compression (kernel 5) */
 
 /* Output bytes reserved per block by cbm_deflate (a BGZF block is at most 65,536 bytes). */
