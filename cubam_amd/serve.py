@@ -255,10 +255,41 @@ def decode_plain(src: BamSource, sp: Span, tags=("CB", "UB"), ws: _decode.Worksp
 
 
 # ------------------------------------------------------------------ the request loop
+@dataclass
+class WriteResult:
+    """The records of a write request that passed its filter, as BGZF blocks on the device (no
+    header, no EOF block: :meth:`to_bam` makes a whole file of them)."""
+
+    blocks: torch.Tensor  # uint8: BGZF blocks back to back
+    payload_bytes: int  # record bytes they hold
+    records: int
+
+    def to_bam(self, header) -> bytes:
+        from cubam_amd.write import EOF_BLOCK, bgzf_compress
+
+        hb = header.to_bytes() if hasattr(header, "to_bytes") else bytes(header)
+        return bgzf_compress(hb).cpu().numpy().tobytes() + self.blocks.cpu().numpy().tobytes() + EOF_BLOCK
+
+
+def write_back(rb: RecordBatch, cells: torch.Tensor, tag: str = "CB", ws: _decode.Workspace | None = None) -> WriteResult:
+    """The records of ``rb`` whose ``tag`` barcode is one of ``cells`` (packed codes, int64; see
+    :meth:`~cubam_amd.bam.Tag.packed`), gathered and BGZF-compressed on the current stream."""
+    from cubam_amd.write import bgzf_compress, gather_records
+
+    codes, ok = rb.tag(tag).packed()
+    keep = ok & torch.isin(codes.view(torch.int64), cells)
+    idx = torch.nonzero(keep).flatten()
+    rec = gather_records(rb.data, rb.offset[idx])
+    n = int(rec.numel())
+    blocks = bgzf_compress(rec, ws=ws) if n else torch.empty(0, dtype=_U8, device=rec.device)
+    return WriteResult(blocks, n, int(idx.numel()))
+
+
 @dataclass(eq=False)
 class Request:
     file: int
     span: Span
+    cells: torch.Tensor | None = None  # a write request: keep these barcodes, return BGZF blocks
     arrival: float = 0.0  # time.perf_counter() when it arrived
     id: int = 0
     result: RecordBatch | None = None
@@ -310,14 +341,20 @@ class Server:
 
     # ---- client side
     def submit(self, file: int, span: Span | None = None, region: tuple[int, int, int] | None = None,
-               arrival: float | None = None) -> Request:
-        """Queue a request for a span (or a region ``(ref, beg, end)``) of file ``file``."""
+               arrival: float | None = None, cells: torch.Tensor | None = None) -> Request:
+        """Queue a request for a span (or a region ``(ref, beg, end)``) of file ``file``.
+
+        With ``cells`` (packed barcode codes, int64 on the device) it is a write request:
+        its result is a :class:`WriteResult`, the records of those cells as BGZF blocks.
+        """
         if span is None:
             if region is None:
                 raise ValueError("a request names a span or a region")
             span = self.sources[file].index.region(*region)
-        req = Request(file, span, time.perf_counter() if arrival is None else arrival)
-        req.est = estimate_bytes(span, len(self.tags))
+        if cells is not None and "CB" not in self.tags:
+            raise ValueError("write requests filter on CB: serve with CB among the tags")
+        req = Request(file, span, cells, time.perf_counter() if arrival is None else arrival)
+        req.est = estimate_bytes(span, len(self.tags)) * (2 if cells is not None else 1)
         if req.est > self.budget:
             raise ValueError(f"request needs about {req.est} device bytes, more than the budget {self.budget}")
         with self._cv:
@@ -404,7 +441,8 @@ class Server:
                         else:
                             r.result = res
                             if self.keep:
-                                nb = result_bytes(res) if len(batch) == 1 else r.est
+                                nb = (result_bytes(res) if len(batch) == 1 else r.est) if isinstance(
+                                    res, RecordBatch) else int(res.blocks.numel())
                                 self._resident[r.id] = (r, nb)
                                 self.resident_bytes += nb
                         r.done_at = now
@@ -418,6 +456,9 @@ class Server:
                 out = [decode_plain(self.sources[r.file], r.span, self.tags, ws)]
             else:
                 out = decode_spans([(self.sources[r.file], r.span) for r in batch], self.tags, ws, pinned)
+            for k, r in enumerate(batch):
+                if r.cells is not None:
+                    out[k] = write_back(out[k], r.cells, ws=ws)
             torch.cuda.current_stream().synchronize()
             return out
         except Exception as e:  # noqa: BLE001
@@ -427,7 +468,10 @@ class Server:
         outs = []
         for r in batch:
             try:
-                outs.append(decode_spans([(self.sources[r.file], r.span)], self.tags, ws, pinned)[0])
+                res = decode_spans([(self.sources[r.file], r.span)], self.tags, ws, pinned)[0]
+                if r.cells is not None:
+                    res = write_back(res, r.cells, ws=ws)
+                outs.append(res)
                 torch.cuda.current_stream().synchronize()
             except Exception as e:  # noqa: BLE001
                 outs.append(e)
@@ -440,12 +484,14 @@ class TraceItem:
     t: float  # arrival, seconds after the start
     file: int
     span: Span
+    cells: torch.Tensor | None = None  # a write request
 
 
 def make_trace(sources: list[BamSource], n: int, rate: float, seed: int = 0, min_bytes: int = 64 << 10,
-               max_bytes: int = 256 << 20) -> list[TraceItem]:
+               max_bytes: int = 256 << 20, write_fraction: float = 0.0, cells=None) -> list[TraceItem]:
     """Poisson arrivals at ``rate`` per second; each request a region of a random file whose
-    compressed span is log-uniform in [min_bytes, max_bytes] (clipped to the file)."""
+    compressed span is log-uniform in [min_bytes, max_bytes] (clipped to the file). A
+    ``write_fraction`` of them are write requests for the barcodes ``cells``."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
     out = []
@@ -463,7 +509,8 @@ def make_trace(sources: list[BamSource], n: int, rate: float, seed: int = 0, min
         if i1 == nblk and ix.coff[-1] + ix.bsize[-1] - c0 < want:  # ran off the end: move the start back
             back = int(np.searchsorted(ix.coff[rb], ix.coff[-1] + ix.bsize[-1] - want, side="left"))
             i0 = max(0, min(i0, back))
-        out.append(TraceItem(float(t[i]), f, ix.span(i0, i1)))
+        w = cells if (write_fraction > 0 and rng.random() < write_fraction) else None
+        out.append(TraceItem(float(t[i]), f, ix.span(i0, i1), w))
     return out
 
 
@@ -475,7 +522,7 @@ def run_trace(server: Server, trace: list[TraceItem]) -> list[Request]:
         delay = t0 + it.t - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        reqs.append(server.submit(it.file, it.span, arrival=t0 + it.t))
+        reqs.append(server.submit(it.file, it.span, arrival=t0 + it.t, cells=it.cells))
     for r in reqs:
         r.done.wait()
     return reqs
@@ -489,7 +536,9 @@ def summarize(reqs: list[Request]) -> dict:
     t_last = max(r.done_at for r in reqs)
     wall = t_last - t_first
     errors = sum(r.error is not None for r in reqs)
+    writes = [r for r in reqs if r.cells is not None]
     return {
+        "write_requests": len(writes),
         "requests": len(reqs),
         "errors": errors,
         "p50_ms": round(float(np.percentile(lat, 50)), 3),

@@ -249,3 +249,54 @@ def test_bad_request_does_not_poison_batch(files):
             r.done.wait()
     assert good.error is None and other.error is None
     assert isinstance(broken.error, ValueError) and "CRC32" in str(broken.error)
+
+
+def test_mixed_read_write_requests(files, oracle):
+    """Write requests in the loop: the records of a barcode group as BGZF blocks, exact against
+    the oracle's records, the same alone and in a mixed batch."""
+    import gzip
+    import zlib
+
+    import torch
+
+    from cubam_amd.serve import Server, decode_spans, make_trace, run_trace, write_back
+
+    srcs = [f[1] for f in files]
+    b, src, data, cols = files[1]  # ATAC-like
+    tb = cols["tag_CB"]
+    cb = [data[o : o + n].tobytes().decode() if t == ord("Z") else "" for t, o, n in
+          zip(tb["type"], tb["offset"], tb["length"])]
+    pool = sorted(set(cb) - {""})
+    keep = pool[::5]
+    cells = torch.tensor([oracle.pack_acgt(c.encode())[0] for c in keep], dtype=torch.int64).view(torch.int64)
+    cells = cells.cuda()
+    trace = [t for t in make_trace(srcs, 30, rate=300.0, seed=12, min_bytes=16 << 10, max_bytes=2 << 20,
+                                   write_fraction=1.0, cells=cells) if t.file == 1]
+    reads = make_trace(srcs, 20, rate=300.0, seed=13, min_bytes=16 << 10, max_bytes=2 << 20)
+    mixed = sorted(trace + reads, key=lambda t: t.t)
+    with Server(srcs, workers=2) as srv:
+        reqs = run_trace(srv, mixed)
+    ks = set(keep)
+    n_checked = 0
+    for r in reqs:
+        assert r.error is None
+        if r.cells is None:
+            continue
+        res = r.result
+        sp = r.span
+        want = b"".join(
+            data[cols["offset"][i] : cols["offset"][i] + 4 + int.from_bytes(
+                data[cols["offset"][i] : cols["offset"][i] + 4].tobytes(), "little")].tobytes()
+            for i in range(sp.rec0, sp.rec1) if cb[i] in ks)
+        blob = res.blocks.cpu().numpy().tobytes()
+        assert (gzip.decompress(blob) if blob else b"") == want and res.payload_bytes == len(want)
+        # blocks cut as htslib cuts them, each with zlib's CRC32
+        o = 0
+        for a in range(0, len(want), 65280):
+            bs = (blob[o + 16] | (blob[o + 17] << 8)) + 1
+            assert int.from_bytes(blob[o + bs - 8 : o + bs - 4], "little") == zlib.crc32(want[a : a + 65280])
+            o += bs
+        alone = write_back(decode_spans([(srcs[1], sp)])[0], cells)
+        assert torch.equal(alone.blocks.cpu(), res.blocks.cpu())
+        n_checked += 1
+    assert n_checked >= 3
