@@ -195,6 +195,92 @@ def batched_bench(n_blocks: int, steps: int, seed: int = 2):
     }
 
 
+def zipf_cells(codes, ok, n_cells: int = 1000, seed: int = 4):
+    """n_cells barcodes (packed codes) drawn without replacement with probability proportional
+    to their read counts, which follow the generator's Zipf(1.1) cell weights."""
+    import numpy as np
+    import torch
+
+    u, cnt = torch.unique(codes.view(torch.int64)[ok], return_counts=True)
+    u, cnt = u.cpu().numpy(), cnt.cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(u.size, size=min(n_cells, u.size), replace=False, p=cnt / cnt.sum())
+    return torch.from_numpy(u[pick]).cuda()
+
+
+def write_bench(buf, n, start, n_ref, ws, header, reps: int = 3):
+    """Write-back: the chunk's records of 1,000 Zipf-selected cells, gathered and BGZF-compressed on
+    the GPU (level-1 matching, dynamic Huffman), checked block by block against zlib."""
+    import zlib
+
+    import numpy as np
+    import torch
+
+    from cubam_amd import _decode
+    from cubam_amd.bam import RecordBatch
+    from cubam_amd.serve import write_back
+
+    inf, chain, off, cols = gpu_step(buf, n, start, n_ref, ws)
+    rb = RecordBatch(inf.data, off, header, cols["fixed"], cols["tags"])
+    codes, ok = rb.tag("CB").packed()
+    cells = zipf_cells(codes, ok)
+    wws = _decode.Workspace()
+    res = write_back(rb, cells, ws=wws)  # warm
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        res = write_back(rb, cells, ws=wws)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    ms = float(np.median(times)) * 1e3
+    # exactness: every block inflates (GPU) to the gathered records, with zlib's CRC32 of its slice
+    from cubam_amd.bgzf import _inflate_blocks
+    from cubam_amd.write import gather_records
+
+    kept = torch.nonzero(ok & torch.isin(codes.view(torch.int64), cells)).flatten()
+    payload = gather_records(rb.data, rb.offset[kept])
+    blob = res.blocks
+    bb = torch.empty(int(blob.numel()) + 64, dtype=torch.uint8, device=blob.device)
+    bb[: blob.numel()].copy_(blob)
+    data2, n2, used, inf2 = _inflate_blocks(bb, int(blob.numel()), True, None, _decode.Workspace())
+    _decode.check_inflated(inf2)
+    exact = n2 == payload.numel() and bool(torch.equal(data2[:n2], payload))
+    host = payload.cpu().numpy()
+    crc_gpu = inf2.crc.cpu().numpy()
+    crc_ok = all(int(crc_gpu[i]) == zlib.crc32(host[a : a + 65280]) for i, a in enumerate(range(0, host.size, 65280)))
+    # and zlib itself on a sample of the blocks
+    raw = blob.cpu().numpy().tobytes()
+    o, k, zl = 0, 0, True
+    while o < len(raw) and k < 64:
+        bs = (raw[o + 16] | (raw[o + 17] << 8)) + 1
+        zl &= zlib.decompress(raw[o + 18 : o + bs - 8], -15) == host[65280 * k : 65280 * (k + 1)].tobytes()
+        o += bs
+        k += 1
+    if not (exact and crc_ok and zl):
+        raise RuntimeError("write-back blocks do not round-trip")
+    z1 = sum(len(zlib.compress(host[a : a + 65280].tobytes(), 1)) for a in range(0, min(host.size, 64 << 20), 65280))
+    sample = min(host.size, 64 << 20)
+    ours = 0
+    o = 0
+    for a in range(0, sample, 65280):
+        bs = (raw[o + 16] | (raw[o + 17] << 8)) + 1
+        ours += bs - 26
+        o += bs
+    return {
+        "workload": "records of 1,000 Zipf-selected cells (CB) of the 1 GiB ATAC chunk, gathered and BGZF-compressed"
+                    " on the GPU (65,280-byte payloads, level-1 greedy matching, dynamic Huffman)",
+        "cells": int(cells.numel()),
+        "records": int(res.records),
+        "payload_bytes": int(res.payload_bytes),
+        "bgzf_bytes": int(blob.numel()),
+        "ms": round(ms, 3),
+        "payload_GBps": round(res.payload_bytes / ms / 1e6, 3),
+        "deflate_over_zlib_level1": round(ours / max(z1, 1), 4),
+        "bit_exact_payload_and_crc32": bool(exact and crc_ok and zl),
+    }
+
+
 SERVE_FILES = ((0, 30), (0, 31), (0, 32), (0, 33), (1, 34), (1, 35), (1, 36), (1, 37))
 
 
@@ -249,7 +335,22 @@ def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, bud
         log(f"rate {row['rate_rps']}/s: plain p50 {row['plain']['p50_ms']} p99 {row['plain']['p99_ms']} ms;"
             f" served p50 {row['served']['p50_ms']} p99 {row['served']['p99_ms']} ms")
         sweep.append(row)
+    # mixed read/write at the plain mode's capacity: a fifth of the requests write the records of
+    # 1,000 Zipf-selected cells back as BGZF blocks
+    from cubam_amd.serve import decode_spans
+
+    atac = next(i for i, (kind, _) in enumerate(SERVE_FILES) if kind == 1)
+    rb = decode_spans([(srcs[atac], srcs[atac].index.span(0, srcs[atac].index.record_blocks().size))])[0]
+    codes, ok = rb.tag("CB").packed()
+    cells = zipf_cells(codes, ok)
+    del rb, codes, ok
+    tr = make_trace(srcs, n_requests, plain_rps, seed=20, write_fraction=0.2, cells=cells)
+    mixed = {"rate_rps": round(plain_rps, 2), "write_fraction": 0.2, "plain": run("plain", tr),
+             "served": run("served", tr)}
+    log(f"mixed read/write: plain p50 {mixed['plain']['p50_ms']} p99 {mixed['plain']['p99_ms']} ms;"
+        f" served p50 {mixed['served']['p50_ms']} p99 {mixed['served']['p99_ms']} ms")
     return {
+        "mixed_read_write": mixed,
         "workload": (f"8 synthetic BAMs in pinned host memory (4 GEX-like x {gex_records} records, 4 ATAC-like x"
                      f" {atac_records}); region requests, compressed span log-uniform 64 KiB-256 MiB"),
         "compressed_bytes": c_total,
@@ -271,6 +372,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-write", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--batched-blocks", type=int, default=500_000)
@@ -366,6 +468,11 @@ def main():
         },
     }
     del buf
+    if not args.no_write:
+        wbuf = _decode.to_device(chunk)
+        res["write_back"] = write_bench(wbuf, n, hdr.size, n_ref, ws, hdr)
+        log(f"write-back: {res['write_back']}")
+        del wbuf
     if not args.no_batched:
         res["batched"] = batched_bench(args.batched_blocks, max(3, min(args.steps, 10)))
     if not args.no_serve:
