@@ -148,29 +148,62 @@ def inflate_chunks(path: str | Path, chunk_bytes: int | None = None, **reader_kw
         yield data[:n]
 
 
+#: Upload the next chunk on a copy stream while the current one decodes (the reference
+#: copies each chunk right before decoding it, src/cubam/bgzf.py:640-646)
+OVERLAP_UPLOAD = True
+
+
 def _inflate(path, chunk_bytes: int | None, prefix: Callable[[], torch.Tensor] | None,
              ws: _decode.Workspace | None = None, **reader_kw):
     """Yields (data, n, inflated) per chunk, each after the bytes ``prefix()`` returns.
 
     As the reference's ``_inflate`` (src/cubam/bgzf.py:625-652): the compressed
     carry (the incomplete last block) stays on the device in front of the next
-    chunk; ``prefix`` is called before each chunk is inflated.
+    chunk; ``prefix`` is called before each chunk is inflated. The reader thread
+    reads chunks ahead into pinned buffers; each chunk's host-to-device copy runs
+    on a copy stream while the chunk before it is decoded, and the carry and the
+    uploaded chunk are joined on the device.
     """
     from cubam_amd._memory import auto_chunk_bytes
 
     dev = _decode.device()
     chunk_bytes = chunk_bytes or auto_chunk_bytes()
     reader = _ChunkReader(path, chunk_bytes, **reader_kw)
+    compute = torch.cuda.current_stream()
+    copier = torch.cuda.Stream() if OVERLAP_UPLOAD else compute
+    items = iter(reader)
     carry = torch.empty(0, dtype=torch.uint8, device=dev)
+
+    def upload():
+        item = next(items, None)
+        if item is None:
+            return None
+        slot, nbytes, eof = item
+        with torch.cuda.stream(copier):
+            stage = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            stage.copy_(reader.host(slot)[:nbytes], non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(copier)
+        return slot, nbytes, eof, stage, done
+
     try:
-        for slot, nbytes, eof in reader:
+        pending = upload()
+        while pending is not None:
+            slot, nbytes, eof, stage, done = pending
+            compute.wait_event(done)
+            stage.record_stream(compute)
             c = int(carry.numel())
             buf = torch.empty(c + nbytes + PAD, dtype=torch.uint8, device=dev)
             if c:
                 buf[:c].copy_(carry)
-            buf[c : c + nbytes].copy_(reader.host(slot)[:nbytes], non_blocking=True)
+            buf[c : c + nbytes].copy_(stage)
+            del stage
+            if OVERLAP_UPLOAD:
+                pending = None if eof else upload()  # the next chunk's copy overlaps this chunk's decode
             data, n, used, inf = _inflate_blocks(buf, c + nbytes, eof, prefix() if prefix else None, ws)
-            reader.release(slot)  # the scan synchronised: the copy is done
+            if not OVERLAP_UPLOAD:
+                pending = None if eof else upload()
+            reader.release(slot)  # the scan synchronised the compute stream, which waited for the copy
             carry = buf[used : c + nbytes].clone()
             del buf
             yield data, n, inf
