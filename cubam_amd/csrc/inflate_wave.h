@@ -75,7 +75,8 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 #define CBM_INFLATE_STATS 0
 #endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
-       STAT_DSTEPS, STAT_LINKIT, STAT_N };
+       STAT_DSTEPS, STAT_LINKIT, STAT_H_STAGE, STAT_H_PARSE, STAT_H_CL, STAT_H_CLDEC, STAT_BLOCKEND, STAT_H_STAGE2,
+       STAT_N };
 
 // matches of a round, per workgroup, in global scratch: a block of at most 65,536 bytes has at
 // most 21,846 matches of 3 bytes or more
@@ -394,6 +395,21 @@ __device__ __forceinline__ bool any_bits(const uint32_t* bm, int a, int b) {
     return false;
 }
 
+// Apply match m (dest | dist << 17 | len << 33) if the bytes it reads are final: none of them
+// (at or after op, the round's output start) is pending in the bitmap. Clears its own bits.
+
+__device__ __forceinline__ bool apply_if_ready(uint8_t* out, uint32_t* pend, uint64_t m, int op) {
+    const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+    const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
+    const int lo = src < op ? op : src, hi = src + (dist < len ? dist : len);
+    if (lo < hi && any_bits(pend, lo, hi)) return false;
+    __threadfence_block();  // read the source after its bits were seen clear
+    lds_copy_match(out, dest, dist, len);
+    __threadfence_block();  // the bytes land before their bits clear
+    mark_bits(pend, dest, len, false);
+    return true;
+}
+
 // ------------------------------------------------------------------ staging
 // Stage payload words [w0, w0 + IN_WORDS) (word = 4 payload bytes; payload byte 0 = in[0])
 // into LDS; bytes at or past `limit` (payload bytes + trailer) read as zero.
@@ -497,6 +513,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             const int wb0 = pos0 >> 5;
             stage<NT>(s, in, wb0, limit);
             __syncthreads();
+            PHASE(STAT_H_STAGE);
             if (tid == 0) {
                 int p = pos0;
                 if (p + 3 > in_bits) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_HEADER;
@@ -535,6 +552,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 s.sv[SV_POS] = p;
             }
             __syncthreads();
+            PHASE(STAT_H_PARSE);
             if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
             const int type = s.sv[SV_TYPE];
             if (type == 0) {
@@ -554,7 +572,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 coop_build<NT>(s, s.lens, 32, DBITS, s.dist, DTAB, TAB_DISTS, s.in);
             } else {
                 const int hlit = s.sv[SV_LEN], hdist = s.sv[SV_SRC];
-                // the code-length code, then the code lengths (serial, thread 0); the
+                // the code-length code, then the code lengths (serial, wavefront 0); the
                 // code-length table goes in the distance table's space
                 bool ok = coop_build<NT>(s, s.lens, 19, CBITS, s.dist, DTAB, TAB_CODES, s.in);
                 if (!ok) {
@@ -562,36 +580,51 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     __syncthreads();
                     break;
                 }
+                PHASE(STAT_H_CL);
                 const int p1 = s.sv[SV_POS];
                 const int wb1 = p1 >> 5;
                 stage<NT>(s, in, wb1, limit);
                 __syncthreads();
-                if (tid == 0) {
-                    int p = p1, n = 0, total = hlit + hdist, err = 0, prev = -1;
-                    int wb = wb1;
+                PHASE(STAT_H_STAGE2);
+                if (tid < CBM_WAVE) {
+                    // wavefront 0 decodes the code lengths in lockstep (every lane computes the same
+                    // values), with the code-length table (7-bit, no subtables) held two entries a
+                    // lane in registers: a lookup is a readlane instead of an LDS round trip
+                    const int lane = tid;
+                    const uint32_t packed = (uint32_t)s.dist[2 * lane] | ((uint32_t)s.dist[2 * lane + 1] << 16);
+                    // the table is in registers before the loop: the loop's LDS stores then need no wait
+                    asm volatile("" ::"v"(packed));
+                    // uniform values, so the loop runs on the scalar unit
+                    const int total = __builtin_amdgcn_readfirstlane(hlit + hdist);
+                    const int wbu = __builtin_amdgcn_readfirstlane(wb1);
+                    int p = __builtin_amdgcn_readfirstlane(p1), n = 0, err = 0, prev = -1;
                     uint64_t v = 0;
                     int avail = 0;
                     while (n < total) {
                         if (avail < 14) {  // a code-length symbol uses at most 7 + 7 bits
-                            if ((p >> 5) - wb + 3 >= S::IN_WORDS) break;  // needs a new staging (rare)
-                            v = peek64(s, p, wb);
+                            if ((p >> 5) - wbu + 3 >= S::IN_WORDS) break;  // needs a new staging (rare)
+                            const uint64_t x = peek64(s, p, wbu);
+                            v = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
+                                (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
                             avail = 64;
                         }
-                        uint32_t e = s.dist[v & 127];
-                        int l = e & 15, sym = (e >> 4) & 0x1ff;
+                        const int idx = (int)(v & 127);
+                        const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)packed, idx >> 1);
+                        const uint32_t e = (idx & 1) ? (pk >> 16) : (pk & 0xffff);
+                        const int l = e & 15, sym = (e >> 4) & 0x1ff;
                         if (!l) { err = 1; break; }
                         p += l;
                         v >>= l;
                         avail -= l;
                         if (sym < 16) {
-                            s.lens[n++] = (uint8_t)sym;
+                            if (lane == 0) s.lens[n] = (uint8_t)sym;
+                            n++;
                             prev = sym;
                         } else {
-                            int rep, xb;
-                            uint8_t val = 0;
+                            int rep, xb, val = 0;
                             if (sym == 16) {
                                 if (prev < 0) { err = 1; break; }
-                                val = (uint8_t)prev;
+                                val = prev;
                                 rep = 3 + (int)(v & 3);
                                 xb = 2;
                             } else if (sym == 17) {
@@ -605,19 +638,24 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             v >>= xb;
                             avail -= xb;
                             if (n + rep > total) { err = 1; break; }
-                            while (rep--) s.lens[n++] = val;
+                            for (int k = lane; k < rep; k += CBM_WAVE) s.lens[n + k] = (uint8_t)val;
+                            n += rep;
                             prev = val;
                         }
                         if (p > in_bits) { err = 1; break; }
                     }
-                    if (!err && n < total) err = 2;  // ran past the staged window
-                    if (!err && s.lens[256] == 0) err = 1;
-                    s.sv[SV_ERR] = err == 1 ? CBM_BLOCK_DEFLATE : CBM_BLOCK_OK;
-                    if (err) s.sv[SV_SITE] = SITE_LENGTHS;
-                    s.sv[SV_CHANGED] = err == 2;
-                    s.sv[SV_POS] = p;
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 0) {
+                        if (!err && n < total) err = 2;  // ran past the staged window
+                        if (!err && s.lens[256] == 0) err = 1;
+                        s.sv[SV_ERR] = err == 1 ? CBM_BLOCK_DEFLATE : CBM_BLOCK_OK;
+                        if (err) s.sv[SV_SITE] = SITE_LENGTHS;
+                        s.sv[SV_CHANGED] = err == 2;
+                        s.sv[SV_POS] = p;
+                    }
                 }
                 __syncthreads();
+                PHASE(STAT_H_CLDEC);
                 if (s.sv[SV_CHANGED]) {
                     // code lengths longer than a staging window: cannot happen for 320 lengths of
                     // at most 7 + 7 bits (4,480 bits) with windows of NT * SEG >= 8,192 bits
@@ -817,26 +855,52 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         for (int u = 0; u < MD_K; u++)
                             if (left & (1u << u)) mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), true);
                         __syncthreads();
-                        for (int step = 0;; step++) {
-                            if (tid == 0) s.sv[SV_F0 + (step + 1) % 3] = 0;
+                        // step 0 by every thread
 #pragma unroll
-                            for (int u = 0; u < MD_K; u++) {
-                                if (!(left & (1u << u))) continue;
-                                const uint64_t m = mm[u];
-                                const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
-                                const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
-                                const int lo = src < op ? op : src, hi = src + (dist < len ? dist : len);
-                                if (lo < hi && any_bits(pend, lo, hi)) continue;
-                                __threadfence_block();  // read the source after its bits were seen clear
-                                left &= ~(1u << u);
-                                lds_copy_match(s.out, dest, dist, len);
-                                __threadfence_block();  // the bytes land before their bits clear
-                                mark_bits(pend, dest, len, false);
-                            }
-                            if (left) s.sv[SV_F0 + step % 3] = 1;
+                        for (int u = 0; u < MD_K; u++)
+                            if ((left & (1u << u)) && apply_if_ready(s.out, pend, mm[u], op)) left &= ~(1u << u);
+                        // the matches still waiting (chains of matches copying matches) are few: they
+                        // are listed after the round's list and one wavefront finishes them, with no
+                        // workgroup barrier between its steps; a longer tail stays with every thread
+                        constexpr int TAIL_K = md_k<NT>();  // matches a lane of the finishing wavefront holds
+                        int rtot;
+                        const int rbase = block_exclusive_sum<NT, int>(__popc(left), s.red, &rtot);
+                        if (rtot > 0 && rtot <= CBM_WAVE * TAIL_K) {
+                            int j = 0;
+#pragma unroll
+                            for (int u = 0; u < MD_K; u++)
+                                if (left & (1u << u)) mlist[mtotal + rbase + j++] = mm[u];
+                            __threadfence_block();
                             __syncthreads();
-                            COUNT(STAT_DSTEPS);
-                            if (!s.sv[SV_F0 + step % 3]) break;
+                            if (tid < CBM_WAVE) {
+                                uint64_t t[TAIL_K];
+                                uint32_t tl = 0;
+#pragma unroll
+                                for (int u = 0; u < TAIL_K; u++) {
+                                    const int k = tid + u * CBM_WAVE;
+                                    t[u] = k < rtot ? __atomic_load_n(&mlist[mtotal + k], __ATOMIC_RELAXED) : 0;
+                                    if (k < rtot) tl |= 1u << u;
+                                }
+                                while (__ballot(tl != 0)) {
+#pragma unroll
+                                    for (int u = 0; u < TAIL_K; u++)
+                                        if ((tl & (1u << u)) && apply_if_ready(s.out, pend, t[u], op)) tl &= ~(1u << u);
+                                    COUNT(STAT_DSTEPS);
+                                }
+                            }
+                        } else if (rtot > 0) {
+                            if (tid == 0) s.sv[SV_F0] = 0;
+                            __syncthreads();
+                            for (int step = 0;; step++) {
+                                if (tid == 0) s.sv[SV_F0 + (step + 1) % 3] = 0;
+#pragma unroll
+                                for (int u = 0; u < MD_K; u++)
+                                    if ((left & (1u << u)) && apply_if_ready(s.out, pend, mm[u], op)) left &= ~(1u << u);
+                                if (left) s.sv[SV_F0 + step % 3] = 1;
+                                __syncthreads();
+                                COUNT(STAT_DSTEPS);
+                                if (!s.sv[SV_F0 + step % 3]) break;
+                            }
                         }
                     } else {
                         if (tid == 0) {
@@ -904,7 +968,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
         }
         // ---- block end: size, store, CRC32
         __syncthreads();
-        PHASE(STAT_HEADER);
+        PHASE(STAT_BLOCKEND);
         int err = s.sv[SV_ERR], site = s.sv[SV_SITE];
         const int op = s.sv[SV_OP];
         if (err == CBM_BLOCK_OK && s.sv[SV_POS] > in_bits) err = CBM_BLOCK_DEFLATE, site = SITE_OVERRUN;
