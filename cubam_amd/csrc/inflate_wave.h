@@ -6,17 +6,23 @@
 // (the speculative self-synchronising decode of Weissenberger and Schmidt,
 // "Massively Parallel Huffman Decoding on GPUs", ICPP 2018). Each round of a
 // Huffman block gives thread t the SEG bits that start at base + t * SEG.
-// Pass A decodes speculatively from there to the first symbol that starts at or
-// after the next segment, and remembers every symbol start it saw. The walks are
-// then linked: thread t's true entry is thread t-1's exit; a thread whose entry
-// changed decodes again from it until it lands on a start it already saw (from
-// there its first walk was right) - repeated until nothing changes.
+// Pass A decodes speculatively, from SEG bits before the segment (so that the walk
+// is most likely on the true boundaries when it enters it) to the first symbol
+// that starts at or after the next segment, and remembers every symbol start it
+// saw inside its segment. The walks are then linked: thread t's true entry is
+// thread t-1's exit; a thread whose entry changed decodes again from it until it
+// lands on a start it already saw (from there its first walk was right) -
+// repeated until nothing changes.
 // Pass B counts each thread's output bytes and matches, a workgroup prefix sum
 // places them, and pass C writes the literals into the LDS output window and the
 // matches, in output order, into a per-workgroup list in global scratch. Pass D
-// applies the matches frontier by frontier: with F the destination of the first
-// match not yet applied, every byte before F is final, so every match whose
-// source ends at or before F is applied at once, and F moves on.
+// applies the matches. Variants whose staging area holds a bit per output byte
+// keep the round's pending bytes in a bitmap there and apply, step by step, every
+// match whose source bytes are no longer pending; the others go frontier by
+// frontier (with F the destination of the first match not yet applied, every
+// byte before F is final).
+// A dynamic block's code lengths are decoded the same way by the 64 lanes of one
+// wavefront, 32 bits a lane.
 // The block's output then leaves LDS in one coalesced store, and its CRC32 is
 // computed from LDS by all threads (slices combined with x^(8n) mod P, the
 // method of zlib's crc32_combine).
@@ -76,7 +82,7 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 #endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
        STAT_DSTEPS, STAT_LINKIT, STAT_H_STAGE, STAT_H_PARSE, STAT_H_CL, STAT_H_CLDEC, STAT_BLOCKEND, STAT_H_STAGE2,
-       STAT_N };
+       STAT_CLWIN, STAT_DYN, STAT_D_MARK, STAT_D_STEP0, STAT_MBIG, STAT_MSUM, STAT_RSUM, STAT_N };
 
 // matches of a round, per workgroup, in global scratch: a block of at most 65,536 bytes has at
 // most 21,846 matches of 3 bytes or more
@@ -115,6 +121,35 @@ __device__ __forceinline__ uint64_t peek64(const S& s, int p, int wbase) {
     v >>= sh;
     if (sh) v |= (uint64_t)hi << (64 - sh);
     return v;
+}
+
+// 32 bits of the staged stream from bit p
+template <class S>
+__device__ __forceinline__ uint32_t peek32(const S& s, int p, int wbase) {
+    const int w = (p >> 5) - wbase;
+    return __builtin_amdgcn_alignbit(s.in[w + 1], s.in[w], p & 31);
+}
+
+// one code-length symbol at bit p, looked up in the code-length table (7-bit, no subtables) held
+// in the distance table's space: its bits (code and extra bits; 0 for an invalid code), the
+// symbol (0..18) and how many lengths it gives
+struct ClSym {
+    int bits, sym, rep;
+};
+constexpr int CL_SEG = 32;  // bits a lane takes: more than the 14 of the longest symbol
+
+template <class S>
+__device__ __forceinline__ ClSym cl_sym(const S& s, int p, int wbase) {
+    const uint32_t v = peek32(s, p, wbase);
+    const uint32_t e = s.dist[v & 127];
+    const int l = e & 15, sym = (e >> 4) & 0x1ff;
+    const uint32_t x = v >> l;
+    ClSym c{l, sym, 1};
+    if (!l) return c;
+    if (sym == 16) c.bits = l + 2, c.rep = 3 + (int)(x & 3);
+    else if (sym == 17) c.bits = l + 3, c.rep = 3 + (int)(x & 7);
+    else if (sym == 18) c.bits = l + 7, c.rep = 11 + (int)(x & 127);
+    return c;
 }
 
 struct Sym {
@@ -395,14 +430,32 @@ __device__ __forceinline__ bool any_bits(const uint32_t* bm, int a, int b) {
     return false;
 }
 
-// Apply match m (dest | dist << 17 | len << 33) if the bytes it reads are final: none of them
-// (at or after op, the round's output start) is pending in the bitmap. Clears its own bits.
-
-__device__ __forceinline__ bool apply_if_ready(uint8_t* out, uint32_t* pend, uint64_t m, int op) {
+// Are the bytes match m (dest | dist << 17 | len << 33) reads final: is none of them (at or after
+// op, the round's output start) pending in the bitmap?
+__device__ __forceinline__ bool match_ready(const uint32_t* pend, uint64_t m, int op) {
     const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
     const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
     const int lo = src < op ? op : src, hi = src + (dist < len ? dist : len);
-    if (lo < hi && any_bits(pend, lo, hi)) return false;
+    return !(lo < hi && any_bits(pend, lo, hi));
+}
+
+// Match m copied by the lanes of a wavefront, a byte a lane: the bytes read, [src, src + len)
+// or the dist bytes before dest, never overlap the bytes written
+__device__ __forceinline__ void wave_copy_match(uint8_t* out, uint64_t m, int lane) {
+    const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+    const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
+    if (dist >= len) {
+        for (int i = lane; i < len; i += CBM_WAVE) out[dest + i] = out[src + i];
+    } else {
+        for (int i = lane; i < len; i += CBM_WAVE) out[dest + i] = out[src + (int)((uint32_t)i % (uint32_t)dist)];
+    }
+}
+
+// Apply match m if its source is final, then clear its destination's bits
+__device__ __forceinline__ bool apply_if_ready(uint8_t* out, uint32_t* pend, uint64_t m, int op) {
+    if (!match_ready(pend, m, op)) return false;
+    const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
+    const int len = (int)((m >> 33) & 0x1ff);
     __threadfence_block();  // read the source after its bits were seen clear
     lds_copy_match(out, dest, dist, len);
     __threadfence_block();  // the bytes land before their bits clear
@@ -515,6 +568,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             __syncthreads();
             PHASE(STAT_H_STAGE);
             if (tid == 0) {
+                __builtin_amdgcn_s_setprio(3);  // the rest of the workgroup waits
                 int p = pos0;
                 if (p + 3 > in_bits) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_HEADER;
                 uint64_t v = peek64(s, p, wb0);
@@ -539,17 +593,17 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     uint32_t hclen = (uint32_t)((v >> 13) & 15) + 4;
                     p += 14;
                     if (hlit > 286 || hdist > 30) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_HEADER;
-                    for (int i = 0; i < 19; i++) s.lens[i] = 0;
-                    for (uint32_t i = 0; i < hclen; i++) {
-                        s.lens[kClOrder[i]] = (uint8_t)(peek64(s, p, wb0) & 7);
-                        p += 3;
-                    }
+                    // the code-length code's lengths, 3 bits each, are read by 19 threads below
+                    s.sv[SV_F0] = (int)hclen;
+                    s.sv[SV_F1] = p;
+                    p += 3 * (int)hclen;
                     s.sv[SV_LEN] = (int)hlit;
                     s.sv[SV_SRC] = (int)hdist;
                 } else if (type == 3) {
                     s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_TYPE;
                 }
                 s.sv[SV_POS] = p;
+                __builtin_amdgcn_s_setprio(0);
             }
             __syncthreads();
             PHASE(STAT_H_PARSE);
@@ -572,8 +626,13 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 coop_build<NT>(s, s.lens, 32, DBITS, s.dist, DTAB, TAB_DISTS, s.in);
             } else {
                 const int hlit = s.sv[SV_LEN], hdist = s.sv[SV_SRC];
-                // the code-length code, then the code lengths (serial, wavefront 0); the
-                // code-length table goes in the distance table's space
+                if (tid < 19) {
+                    const int h = s.sv[SV_F0], q = s.sv[SV_F1] + 3 * tid;
+                    s.lens[kClOrder[tid]] = tid < h ? (uint8_t)(peek32(s, q, wb0) & 7) : 0;
+                }
+                __syncthreads();
+                // the code-length code, then the code lengths (wavefront 0); the code-length
+                // table goes in the distance table's space
                 bool ok = coop_build<NT>(s, s.lens, 19, CBITS, s.dist, DTAB, TAB_CODES, s.in);
                 if (!ok) {
                     if (tid == 0) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_CLCODE;
@@ -581,77 +640,150 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     break;
                 }
                 PHASE(STAT_H_CL);
+                COUNT(STAT_DYN);
                 const int p1 = s.sv[SV_POS];
                 const int wb1 = p1 >> 5;
                 stage<NT>(s, in, wb1, limit);
                 __syncthreads();
                 PHASE(STAT_H_STAGE2);
                 if (tid < CBM_WAVE) {
-                    // wavefront 0 decodes the code lengths in lockstep (every lane computes the same
-                    // values), with the code-length table (7-bit, no subtables) held two entries a
-                    // lane in registers: a lookup is a readlane instead of an LDS round trip
+                    // wavefront 0 decodes the code lengths, CL_SEG bits a lane: each lane walks its
+                    // bits from their first bit, the walks are linked as the rounds link theirs (a
+                    // lane whose entry moves walks again until it meets a symbol start it saw), then
+                    // the lanes count their lengths and write them at offsets a scan gives. Up to
+                    // 64 * CL_SEG bits a pass; the lengths of a block take at most 316 * 14 bits.
+                    __builtin_amdgcn_s_setprio(3);  // the rest of the workgroup waits
                     const int lane = tid;
-                    const uint32_t packed = (uint32_t)s.dist[2 * lane] | ((uint32_t)s.dist[2 * lane + 1] << 16);
-                    // the table is in registers before the loop: the loop's LDS stores then need no wait
-                    asm volatile("" ::"v"(packed));
-                    // uniform values, so the loop runs on the scalar unit
-                    const int total = __builtin_amdgcn_readfirstlane(hlit + hdist);
-                    const int wbu = __builtin_amdgcn_readfirstlane(wb1);
-                    int p = __builtin_amdgcn_readfirstlane(p1), n = 0, err = 0, prev = -1;
-                    uint64_t v = 0;
-                    int avail = 0;
-                    while (n < total) {
-                        if (avail < 14) {  // a code-length symbol uses at most 7 + 7 bits
-                            if ((p >> 5) - wbu + 3 >= S::IN_WORDS) break;  // needs a new staging (rare)
-                            const uint64_t x = peek64(s, p, wbu);
-                            v = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
-                                (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
-                            avail = 64;
+                    const int total = hlit + hdist;
+                    int wbase = p1, n0 = 0, prev0 = -1, err = 0, endp = -1;
+                    while (true) {
+                        COUNT(STAT_CLWIN);
+                        // every symbol start a pass decodes lies below wbase + 64 * CL_SEG
+                        if (((wbase + CBM_WAVE * CL_SEG) >> 5) - wb1 + 2 >= S::IN_WORDS) {
+                            err = 2;
+                            break;
                         }
-                        const int idx = (int)(v & 127);
-                        const uint32_t pk = (uint32_t)__builtin_amdgcn_readlane((int)packed, idx >> 1);
-                        const uint32_t e = (idx & 1) ? (pk >> 16) : (pk & 0xffff);
-                        const int l = e & 15, sym = (e >> 4) & 0x1ff;
-                        if (!l) { err = 1; break; }
-                        p += l;
-                        v >>= l;
-                        avail -= l;
-                        if (sym < 16) {
-                            if (lane == 0) s.lens[n] = (uint8_t)sym;
-                            n++;
-                            prev = sym;
-                        } else {
-                            int rep, xb, val = 0;
-                            if (sym == 16) {
-                                if (prev < 0) { err = 1; break; }
-                                val = prev;
-                                rep = 3 + (int)(v & 3);
-                                xb = 2;
-                            } else if (sym == 17) {
-                                rep = 3 + (int)(v & 7);
-                                xb = 3;
-                            } else {
-                                rep = 11 + (int)(v & 127);
-                                xb = 7;
+                        const int a = wbase + lane * CL_SEG, aend = a + CL_SEG;
+                        uint32_t seen = 0;
+                        int p = a, bad = 0;
+                        while (p < aend) {
+                            seen |= 1u << (p - a);
+                            const ClSym c = cl_sym(s, p, wb1);
+                            if (!c.bits || p + c.bits > in_bits) {
+                                bad = 1;
+                                break;
                             }
-                            p += xb;
-                            v >>= xb;
-                            avail -= xb;
-                            if (n + rep > total) { err = 1; break; }
-                            for (int k = lane; k < rep; k += CBM_WAVE) s.lens[n + k] = (uint8_t)val;
-                            n += rep;
-                            prev = val;
+                            p += c.bits;
                         }
-                        if (p > in_bits) { err = 1; break; }
+                        // p: the first symbol start at or past aend, or the start of a symbol that
+                        // is invalid or runs past the input (bad)
+                        const int a_exit = p, a_bad = bad;
+                        int entry = a;
+                        while (true) {
+                            int want = __shfl_up(p, 1, CBM_WAVE), wbad = __shfl_up(bad, 1, CBM_WAVE);
+                            if (lane == 0) want = wbase, wbad = 0;
+                            bool changed = false;
+                            if (!wbad && want != entry) {
+                                int q = want, nb = 0;
+                                while (q < aend) {
+                                    if (q >= a && ((seen >> (q - a)) & 1u)) {
+                                        q = a_exit;
+                                        nb = a_bad;
+                                        break;
+                                    }
+                                    const ClSym c = cl_sym(s, q, wb1);
+                                    if (!c.bits || q + c.bits > in_bits) {
+                                        nb = 1;
+                                        break;
+                                    }
+                                    q += c.bits;
+                                }
+                                changed = q != p || nb != bad;
+                                p = q;
+                                bad = nb;
+                                entry = want;
+                            }
+                            if (!__ballot(changed)) break;
+                        }
+                        // lanes up to the first that stopped on a bad symbol hold the true walk
+                        const uint64_t badm = __ballot(bad);
+                        const int fb = badm ? __ffsll((unsigned long long)badm) - 1 : CBM_WAVE;
+                        const bool alive = lane <= fb;
+                        int cnt = 0, last = -1;
+                        if (alive) {
+                            for (int q = entry; q < p;) {
+                                const ClSym c = cl_sym(s, q, wb1);
+                                if (!c.bits) break;  // not met: the symbols before p are valid
+                                cnt += c.rep;
+                                last = c.sym < 16 ? c.sym : c.sym == 16 ? last : 0;
+                                q += c.bits;
+                            }
+                        }
+                        // offsets (a sum over the lanes before) and the length a leading 16 repeats
+                        int inc = cnt, key = last >= 0 ? ((lane + 1) << 5) | last : 0;
+#pragma unroll
+                        for (int d = 1; d < CBM_WAVE; d <<= 1) {
+                            const int o = __shfl_up(inc, d, CBM_WAVE), k = __shfl_up(key, d, CBM_WAVE);
+                            if (lane >= d) inc += o, key = k > key ? k : key;
+                        }
+                        int kex = __shfl_up(key, 1, CBM_WAVE);
+                        if (lane == 0) kex = 0;
+                        int n = n0 + inc - cnt, prev = kex ? (kex & 31) : prev0, myerr = 0, myend = -1;
+                        if (alive && n < total) {
+                            int q = entry;
+                            while (q < p && n < total) {
+                                const ClSym c = cl_sym(s, q, wb1);
+                                int val = 0;
+                                if (!c.bits) {
+                                    myerr = 1;
+                                    break;
+                                }
+                                if (c.sym < 16) {
+                                    val = c.sym;
+                                } else if (c.sym == 16) {
+                                    if (prev < 0) {
+                                        myerr = 1;  // a repeat with no length before it
+                                        break;
+                                    }
+                                    val = prev;
+                                }
+                                if (n + c.rep > total) {
+                                    myerr = 1;  // a repeat past the last length
+                                    break;
+                                }
+                                for (int k = 0; k < c.rep; k++) s.lens[n + k] = (uint8_t)val;
+                                n += c.rep;
+                                prev = val;
+                                q += c.bits;
+                            }
+                            if (!myerr) {
+                                if (n == total) myend = q;
+                                else if (bad) myerr = 1;  // the lengths end on a bad symbol
+                            }
+                        }
+                        if (__ballot(myerr)) {
+                            err = 1;
+                            break;
+                        }
+                        const uint64_t em = __ballot(myend >= 0);
+                        if (em) {
+                            endp = __shfl(myend, __ffsll((unsigned long long)em) - 1, CBM_WAVE);
+                            break;
+                        }
+                        // every lane's walk is true and the lengths go on past this pass
+                        n0 = __shfl(n0 + inc, CBM_WAVE - 1, CBM_WAVE);
+                        const int k63 = __shfl(key, CBM_WAVE - 1, CBM_WAVE);
+                        prev0 = k63 ? (k63 & 31) : prev0;
+                        wbase = __shfl(p, CBM_WAVE - 1, CBM_WAVE);
                     }
                     __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_s_setprio(0);
                     if (lane == 0) {
-                        if (!err && n < total) err = 2;  // ran past the staged window
                         if (!err && s.lens[256] == 0) err = 1;
                         s.sv[SV_ERR] = err == 1 ? CBM_BLOCK_DEFLATE : CBM_BLOCK_OK;
                         if (err) s.sv[SV_SITE] = SITE_LENGTHS;
                         s.sv[SV_CHANGED] = err == 2;
-                        s.sv[SV_POS] = p;
+                        s.sv[SV_POS] = endp;
                     }
                 }
                 __syncthreads();
@@ -688,28 +820,44 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 const int a = base + tid * SEG;         // speculative start
                 const int aend = a + SEG;
                 const bool active = a < in_bits;        // thread 0 is always active
-                // pass A
+                // pass A. Threads after the first start OVL bits before their segment, so that the
+                // walk has most likely fallen onto the true symbol boundaries when it enters the
+                // segment (then the link below finds the entry among the starts it saw at once);
+                // only starts inside the segment are remembered. A walk that stops (end of block
+                // or a bad code) before its segment starts again from the segment start.
+#ifndef CBM_INFLATE_OVL_HALVES
+#define CBM_INFLATE_OVL_HALVES 2
+#endif
+                constexpr int OVL = SEG * CBM_INFLATE_OVL_HALVES / 2;
                 uint32_t seen[SEG / 32];
 #pragma unroll
                 for (int k = 0; k < SEG / 32; k++) seen[k] = 0;
                 int p = a, stop = STOP_NONE;
                 if (active) {
                     Win w;
-                    win_at(w, s, a, wb);
+                    win_at(w, s, a - OVL > base ? a - OVL : base, wb);
                     while (w.p < aend) {
-                        int r = w.p - a;
+                        const int r = w.p - a;  // negative in the overlap: marks nothing
 #pragma unroll
                         for (int k = 0; k < SEG / 32; k++)
                             if ((r >> 5) == k) seen[k] |= 1u << (r & 31);
+                        const int at = w.p;
                         Sym y = win_next(w, s, wb);
+                        const bool halt = y.kind == 2 || y.kind == 3 || w.p > in_bits;
+                        if (halt && at < a) {
+                            win_at(w, s, a, wb);
+                            continue;
+                        }
                         if (y.kind == 2) { stop = STOP_EOB; break; }
-                        if (y.kind == 3 || w.p > in_bits) { stop = STOP_BAD; break; }
+                        if (halt) { stop = STOP_BAD; break; }
                     }
                     p = w.p;
                 }
                 s.exit_[tid] = p;
                 s.stop_[tid] = active ? stop : STOP_BAD;
-                int entry = a;
+                // the walk's exit is the exit from a only if it passed through a; else the first
+                // link step walks again from the predecessor's exit
+                int entry = (seen[0] & 1u) ? a : -1;
                 // a later walk that lands on a start of this first walk takes this walk's end
                 const int a_exit = p, a_stop = stop;
                 __syncthreads();
@@ -836,6 +984,9 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // match: every byte before F is final. Rounds with more matches than the threads
                 // hold go through one wavefront in list order.
                 constexpr int MD_K = md_k<NT>();
+#if CBM_INFLATE_STATS
+                if (tid == 0) acc[STAT_MSUM] += mtotal, acc[STAT_MBIG] += mtotal > MD_K * NT;
+#endif
                 if (mtotal <= MD_K * NT) {
                     uint64_t mm[MD_K];
                     uint32_t left = 0;
@@ -855,6 +1006,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         for (int u = 0; u < MD_K; u++)
                             if (left & (1u << u)) mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), true);
                         __syncthreads();
+                        PHASE(STAT_D_MARK);
                         // step 0 by every thread
 #pragma unroll
                         for (int u = 0; u < MD_K; u++)
@@ -865,6 +1017,10 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         constexpr int TAIL_K = md_k<NT>();  // matches a lane of the finishing wavefront holds
                         int rtot;
                         const int rbase = block_exclusive_sum<NT, int>(__popc(left), s.red, &rtot);
+                        PHASE(STAT_D_STEP0);
+#if CBM_INFLATE_STATS
+                        if (tid == 0) acc[STAT_RSUM] += rtot;
+#endif
                         if (rtot > 0 && rtot <= CBM_WAVE * TAIL_K) {
                             int j = 0;
 #pragma unroll
@@ -873,6 +1029,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             __threadfence_block();
                             __syncthreads();
                             if (tid < CBM_WAVE) {
+                                __builtin_amdgcn_s_setprio(3);  // the rest of the workgroup waits
                                 uint64_t t[TAIL_K];
                                 uint32_t tl = 0;
 #pragma unroll
@@ -881,21 +1038,60 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                     t[u] = k < rtot ? __atomic_load_n(&mlist[mtotal + k], __ATOMIC_RELAXED) : 0;
                                     if (k < rtot) tl |= 1u << u;
                                 }
+                                // each step: the matches whose sources are final, then each of
+                                // them copied by the whole wavefront, then their bits cleared
                                 while (__ballot(tl != 0)) {
+                                    uint32_t rdy = 0;
 #pragma unroll
                                     for (int u = 0; u < TAIL_K; u++)
-                                        if ((tl & (1u << u)) && apply_if_ready(s.out, pend, t[u], op)) tl &= ~(1u << u);
+                                        if ((tl & (1u << u)) && match_ready(pend, t[u], op)) rdy |= 1u << u;
+                                    tl &= ~rdy;
+                                    __threadfence_block();  // read the sources after their bits were seen clear
+#pragma unroll
+                                    for (int u = 0; u < TAIL_K; u++) {
+                                        uint64_t bal = __ballot(rdy & (1u << u));
+                                        while (bal) {
+                                            const int from = __ffsll((unsigned long long)bal) - 1;
+                                            bal &= bal - 1;
+                                            wave_copy_match(s.out, __shfl(t[u], from, CBM_WAVE), tid);
+                                        }
+                                    }
+                                    __threadfence_block();  // the bytes land before their bits clear
+#pragma unroll
+                                    for (int u = 0; u < TAIL_K; u++)
+                                        if (rdy & (1u << u))
+                                            mark_bits(pend, (int)(t[u] & 0x1ffff), (int)((t[u] >> 33) & 0x1ff), false);
                                     COUNT(STAT_DSTEPS);
                                 }
+                                __builtin_amdgcn_s_setprio(0);
                             }
                         } else if (rtot > 0) {
                             if (tid == 0) s.sv[SV_F0] = 0;
                             __syncthreads();
                             for (int step = 0;; step++) {
                                 if (tid == 0) s.sv[SV_F0 + (step + 1) % 3] = 0;
+                                // as in the single-wavefront tail: each wavefront copies the ready
+                                // matches of its lanes together, a byte a lane
+                                uint32_t rdy = 0;
 #pragma unroll
                                 for (int u = 0; u < MD_K; u++)
-                                    if ((left & (1u << u)) && apply_if_ready(s.out, pend, mm[u], op)) left &= ~(1u << u);
+                                    if ((left & (1u << u)) && match_ready(pend, mm[u], op)) rdy |= 1u << u;
+                                left &= ~rdy;
+                                __threadfence_block();
+#pragma unroll
+                                for (int u = 0; u < MD_K; u++) {
+                                    uint64_t bal = __ballot(rdy & (1u << u));
+                                    while (bal) {
+                                        const int from = __ffsll((unsigned long long)bal) - 1;
+                                        bal &= bal - 1;
+                                        wave_copy_match(s.out, __shfl(mm[u], from, CBM_WAVE), tid % CBM_WAVE);
+                                    }
+                                }
+                                __threadfence_block();
+#pragma unroll
+                                for (int u = 0; u < MD_K; u++)
+                                    if (rdy & (1u << u))
+                                        mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), false);
                                 if (left) s.sv[SV_F0 + step % 3] = 1;
                                 __syncthreads();
                                 COUNT(STAT_DSTEPS);
