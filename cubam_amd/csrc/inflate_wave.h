@@ -80,6 +80,14 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 #ifndef CBM_INFLATE_STATS
 #define CBM_INFLATE_STATS 0
 #endif
+// passes of match forwarding in pass D, and the bits (in half segments) a speculative walk
+// starts before its segment
+#ifndef CBM_INFLATE_FWD_PASSES
+#define CBM_INFLATE_FWD_PASSES 2
+#endif
+#ifndef CBM_INFLATE_OVL_HALVES
+#define CBM_INFLATE_OVL_HALVES 2
+#endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
        STAT_DSTEPS, STAT_LINKIT, STAT_H_STAGE, STAT_H_PARSE, STAT_H_CL, STAT_H_CLDEC, STAT_BLOCKEND, STAT_H_STAGE2,
        STAT_CLWIN, STAT_DYN, STAT_D_MARK, STAT_D_STEP0, STAT_MBIG, STAT_MSUM, STAT_RSUM, STAT_N };
@@ -430,6 +438,11 @@ __device__ __forceinline__ bool any_bits(const uint32_t* bm, int a, int b) {
     return false;
 }
 
+// pass D, per wavefront: more lanes with ready matches than this and each lane copies its own;
+// idle polls (with a short sleep each) before a round is given up as bad
+constexpr int D_LANE_COPY = 8;
+constexpr int D_IDLE_MAX = 1 << 16;
+
 // Are the bytes match m (dest | dist << 17 | len << 33) reads final: is none of them (at or after
 // op, the round's output start) pending in the bitmap?
 __device__ __forceinline__ bool match_ready(const uint32_t* pend, uint64_t m, int op) {
@@ -449,18 +462,6 @@ __device__ __forceinline__ void wave_copy_match(uint8_t* out, uint64_t m, int la
     } else {
         for (int i = lane; i < len; i += CBM_WAVE) out[dest + i] = out[src + (int)((uint32_t)i % (uint32_t)dist)];
     }
-}
-
-// Apply match m if its source is final, then clear its destination's bits
-__device__ __forceinline__ bool apply_if_ready(uint8_t* out, uint32_t* pend, uint64_t m, int op) {
-    if (!match_ready(pend, m, op)) return false;
-    const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff);
-    const int len = (int)((m >> 33) & 0x1ff);
-    __threadfence_block();  // read the source after its bits were seen clear
-    lds_copy_match(out, dest, dist, len);
-    __threadfence_block();  // the bytes land before their bits clear
-    mark_bits(pend, dest, len, false);
-    return true;
 }
 
 // ------------------------------------------------------------------ staging
@@ -825,9 +826,6 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // segment (then the link below finds the entry among the starts it saw at once);
                 // only starts inside the segment are remembered. A walk that stops (end of block
                 // or a bad code) before its segment starts again from the segment start.
-#ifndef CBM_INFLATE_OVL_HALVES
-#define CBM_INFLATE_OVL_HALVES 2
-#endif
                 constexpr int OVL = SEG * CBM_INFLATE_OVL_HALVES / 2;
                 uint32_t seen[SEG / 32];
 #pragma unroll
@@ -999,6 +997,55 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     if constexpr (S::IN_WORDS >= 2048 + 8) {
                         uint32_t* pend = s.in;
                         const int w0 = op >> 5, w1 = (op + total + 31) >> 5;
+                        // match forwarding: a match that reads only bytes another match of the
+                        // round writes, both without overlap (dist >= len), reads that match's
+                        // source instead (its dist grows by the other's). Each pass looks the
+                        // covering match up (binary search over the round's destinations, kept as
+                        // 16-bit offsets in a free part of the staging area) in the list as the
+                        // last pass left it, so chains shorten like pointer jumping; pass D then
+                        // needs fewer steps.
+                        const int need = (mtotal + 1) / 2;
+                        const int tabw = w0 >= need ? 0 : S::IN_WORDS - w1 >= need ? w1 : -1;
+                        if (tabw >= 0 && mtotal > 1) {
+                            uint16_t* dst16 = (uint16_t*)(s.in + tabw);
+#pragma unroll
+                            for (int u = 0; u < MD_K; u++)
+                                if (left & (1u << u)) dst16[tid + u * NT] = (uint16_t)((int)(mm[u] & 0x1ffff) - op);
+                            __syncthreads();
+                            for (int pass = 0; pass < CBM_INFLATE_FWD_PASSES; pass++) {
+                                uint64_t nm[MD_K];
+#pragma unroll
+                                for (int u = 0; u < MD_K; u++) {
+                                    nm[u] = mm[u];
+                                    if (!(left & (1u << u))) continue;
+                                    const int d = (int)(mm[u] & 0x1ffff), dist = (int)((mm[u] >> 17) & 0xffff);
+                                    const int len = (int)((mm[u] >> 33) & 0x1ff), src = d - dist;
+                                    if (dist < len || src < op) continue;
+                                    // the last match (before this one) whose destination starts at or before src
+                                    int lo = 0, hi = tid + u * NT;
+                                    while (lo < hi) {
+                                        const int mid = (lo + hi) >> 1;
+                                        if ((int)dst16[mid] <= src - op) lo = mid + 1;
+                                        else hi = mid;
+                                    }
+                                    if (lo == 0) continue;
+                                    const uint64_t m2 = __atomic_load_n(&mlist[lo - 1], __ATOMIC_RELAXED);
+                                    const int d2 = (int)(m2 & 0x1ffff), dist2 = (int)((m2 >> 17) & 0xffff);
+                                    const int len2 = (int)((m2 >> 33) & 0x1ff);
+                                    if (dist2 >= len2 && src + len <= d2 + len2)
+                                        nm[u] = (uint64_t)d | ((uint64_t)(dist + dist2) << 17) | ((uint64_t)len << 33);
+                                }
+                                __syncthreads();  // every read of the list by this pass is done
+#pragma unroll
+                                for (int u = 0; u < MD_K; u++)
+                                    if (nm[u] != mm[u]) {
+                                        mm[u] = nm[u];
+                                        mlist[tid + u * NT] = nm[u];
+                                    }
+                                __threadfence_block();
+                                __syncthreads();
+                            }
+                        }
                         for (int i = w0 + tid; i < w1; i += NT) pend[i] = 0;
                         if (tid == 0) s.sv[SV_F0] = 0;
                         __syncthreads();
@@ -1007,77 +1054,41 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             if (left & (1u << u)) mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), true);
                         __syncthreads();
                         PHASE(STAT_D_MARK);
-                        // step 0 by every thread
-#pragma unroll
-                        for (int u = 0; u < MD_K; u++)
-                            if ((left & (1u << u)) && apply_if_ready(s.out, pend, mm[u], op)) left &= ~(1u << u);
-                        // the matches still waiting (chains of matches copying matches) are few: they
-                        // are listed after the round's list and one wavefront finishes them, with no
-                        // workgroup barrier between its steps; a longer tail stays with every thread
-                        constexpr int TAIL_K = md_k<NT>();  // matches a lane of the finishing wavefront holds
-                        int rtot;
-                        const int rbase = block_exclusive_sum<NT, int>(__popc(left), s.red, &rtot);
-                        PHASE(STAT_D_STEP0);
-#if CBM_INFLATE_STATS
-                        if (tid == 0) acc[STAT_RSUM] += rtot;
-#endif
-                        if (rtot > 0 && rtot <= CBM_WAVE * TAIL_K) {
-                            int j = 0;
+                        // every wavefront applies its lanes' matches as their sources become final,
+                        // with no workgroup barrier between steps: the unapplied match with the
+                        // lowest destination always has a final source (every source lies before its
+                        // destination), so some wavefront always progresses. A step with many ready
+                        // lanes lets each lane copy its own matches; otherwise the wavefront copies
+                        // the ready matches one after another, a byte a lane. A wavefront with nothing
+                        // ready sleeps briefly; one that waits past a bound reports the block bad
+                        // instead of spinning on (which no valid round reaches).
+                        int idle = 0;
+                        while (__ballot(left != 0)) {
+                            uint32_t rdy = 0;
 #pragma unroll
                             for (int u = 0; u < MD_K; u++)
-                                if (left & (1u << u)) mlist[mtotal + rbase + j++] = mm[u];
-                            __threadfence_block();
-                            __syncthreads();
-                            if (tid < CBM_WAVE) {
-                                __builtin_amdgcn_s_setprio(3);  // the rest of the workgroup waits
-                                uint64_t t[TAIL_K];
-                                uint32_t tl = 0;
-#pragma unroll
-                                for (int u = 0; u < TAIL_K; u++) {
-                                    const int k = tid + u * CBM_WAVE;
-                                    t[u] = k < rtot ? __atomic_load_n(&mlist[mtotal + k], __ATOMIC_RELAXED) : 0;
-                                    if (k < rtot) tl |= 1u << u;
-                                }
-                                // each step: the matches whose sources are final, then each of
-                                // them copied by the whole wavefront, then their bits cleared
-                                while (__ballot(tl != 0)) {
-                                    uint32_t rdy = 0;
-#pragma unroll
-                                    for (int u = 0; u < TAIL_K; u++)
-                                        if ((tl & (1u << u)) && match_ready(pend, t[u], op)) rdy |= 1u << u;
-                                    tl &= ~rdy;
-                                    __threadfence_block();  // read the sources after their bits were seen clear
-#pragma unroll
-                                    for (int u = 0; u < TAIL_K; u++) {
-                                        uint64_t bal = __ballot(rdy & (1u << u));
-                                        while (bal) {
-                                            const int from = __ffsll((unsigned long long)bal) - 1;
-                                            bal &= bal - 1;
-                                            wave_copy_match(s.out, __shfl(t[u], from, CBM_WAVE), tid);
-                                        }
+                                if ((left & (1u << u)) && match_ready(pend, mm[u], op)) rdy |= 1u << u;
+                            const uint64_t lanes = __ballot(rdy != 0);
+                            if (!lanes) {
+                                if (++idle > D_IDLE_MAX) {
+                                    if ((tid % CBM_WAVE) == 0) {
+                                        atomicOr(&s.sv[SV_ERR], CBM_BLOCK_DEFLATE);
+                                        s.sv[SV_SITE] = SITE_ROUND;
                                     }
-                                    __threadfence_block();  // the bytes land before their bits clear
-#pragma unroll
-                                    for (int u = 0; u < TAIL_K; u++)
-                                        if (rdy & (1u << u))
-                                            mark_bits(pend, (int)(t[u] & 0x1ffff), (int)((t[u] >> 33) & 0x1ff), false);
-                                    COUNT(STAT_DSTEPS);
+                                    break;
                                 }
-                                __builtin_amdgcn_s_setprio(0);
+                                __builtin_amdgcn_s_sleep(1);
+                                continue;
                             }
-                        } else if (rtot > 0) {
-                            if (tid == 0) s.sv[SV_F0] = 0;
-                            __syncthreads();
-                            for (int step = 0;; step++) {
-                                if (tid == 0) s.sv[SV_F0 + (step + 1) % 3] = 0;
-                                // as in the single-wavefront tail: each wavefront copies the ready
-                                // matches of its lanes together, a byte a lane
-                                uint32_t rdy = 0;
+                            left &= ~rdy;
+                            __threadfence_block();  // read the sources after their bits were seen clear
+                            if (__popcll(lanes) > D_LANE_COPY) {
 #pragma unroll
                                 for (int u = 0; u < MD_K; u++)
-                                    if ((left & (1u << u)) && match_ready(pend, mm[u], op)) rdy |= 1u << u;
-                                left &= ~rdy;
-                                __threadfence_block();
+                                    if (rdy & (1u << u))
+                                        lds_copy_match(s.out, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 17) & 0xffff),
+                                                       (int)((mm[u] >> 33) & 0x1ff));
+                            } else {
 #pragma unroll
                                 for (int u = 0; u < MD_K; u++) {
                                     uint64_t bal = __ballot(rdy & (1u << u));
@@ -1087,17 +1098,16 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                         wave_copy_match(s.out, __shfl(mm[u], from, CBM_WAVE), tid % CBM_WAVE);
                                     }
                                 }
-                                __threadfence_block();
-#pragma unroll
-                                for (int u = 0; u < MD_K; u++)
-                                    if (rdy & (1u << u))
-                                        mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), false);
-                                if (left) s.sv[SV_F0 + step % 3] = 1;
-                                __syncthreads();
-                                COUNT(STAT_DSTEPS);
-                                if (!s.sv[SV_F0 + step % 3]) break;
                             }
+                            __threadfence_block();  // the bytes land before their bits clear
+#pragma unroll
+                            for (int u = 0; u < MD_K; u++)
+                                if (rdy & (1u << u))
+                                    mark_bits(pend, (int)(mm[u] & 0x1ffff), (int)((mm[u] >> 33) & 0x1ff), false);
+                            COUNT(STAT_DSTEPS);
                         }
+                        __syncthreads();
+                        PHASE(STAT_D_STEP0);
                     } else {
                         if (tid == 0) {
                             s.sv[SV_F0] = mtotal ? (int)(mm[0] & 0x1ffff) : 0x7fffffff;
