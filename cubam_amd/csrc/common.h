@@ -45,34 +45,67 @@ __device__ __forceinline__ bool ref_consuming(uint32_t op) {
 // reference ids in [-1, n_ref), positions >= -1, l_read_name >= 2, flag < 4096,
 // l_seq <= 4 Mi, the fields fit block_size, a NUL-terminated printable name without
 // '@', CIGAR ops <= 8 and bin == reg2bin(pos, pos + max(rlen, 1)) for mapped positions.
-__device__ inline bool is_record(const uint8_t* b, int64_t n, int64_t o, int n_ref) {
+// The test reads bytes through an accessor A (u8, u16, u32 at a data offset): GlobalBytes reads
+// the data in place; the record walk reads a copy staged in LDS.
+struct GlobalBytes {
+    const uint8_t* b;
+    __device__ __forceinline__ uint32_t u8(int64_t o) const { return b[o]; }
+    __device__ __forceinline__ uint32_t u16(int64_t o) const { return ld_u16(b, o); }
+    __device__ __forceinline__ uint32_t u32(int64_t o) const { return ld_u32(b, o); }
+};
+
+// the fixed-field part of the test; on success the fields the rest needs
+struct RecHead {
+    int32_t pos;
+    uint32_t lrn, ncig;
+};
+template <class A>
+__device__ __forceinline__ bool record_head(const A& a, int64_t n, int64_t o, int n_ref, RecHead& h) {
     if (o < 0 || o + 36 > n) return false;
-    int32_t bs = (int32_t)ld_u32(b, o);
+    int32_t bs = (int32_t)a.u32(o);
     if (bs < 33 || bs > (1 << 22) || o + 4 + bs > n) return false;
-    int32_t ref = (int32_t)ld_u32(b, o + 4), pos = (int32_t)ld_u32(b, o + 8);
-    int32_t nref = (int32_t)ld_u32(b, o + 24), npos = (int32_t)ld_u32(b, o + 28);
+    int32_t ref = (int32_t)a.u32(o + 4), pos = (int32_t)a.u32(o + 8);
+    int32_t nref = (int32_t)a.u32(o + 24), npos = (int32_t)a.u32(o + 28);
     if (ref < -1 || ref >= n_ref || pos < -1 || nref < -1 || nref >= n_ref || npos < -1) return false;
-    uint32_t lrn = b[o + 12], ncig = ld_u16(b, o + 16), flag = ld_u16(b, o + 18);
-    uint32_t lseq = ld_u32(b, o + 20);
+    uint32_t lrn = a.u8(o + 12), ncig = a.u16(o + 16), flag = a.u16(o + 18);
+    uint32_t lseq = a.u32(o + 20);
     if (lrn < 2 || flag >= 4096 || lseq > (1u << 22)) return false;
     if (32LL + lrn + 4LL * ncig + (lseq + 1) / 2 + lseq > bs) return false;
+    if (a.u8(o + 36 + lrn - 1) != 0) return false;
+    h.pos = pos;
+    h.lrn = lrn;
+    h.ncig = ncig;
+    return true;
+}
+
+// bin == reg2bin(pos, pos + max(rlen, 1)) for a mapped position
+template <class A>
+__device__ __forceinline__ bool record_bin_ok(const A& a, int64_t o, int32_t pos, int64_t rlen) {
+    if (pos < 0) return true;
+    int end = pos + (int)(rlen > 0 ? rlen : 1);
+    return a.u16(o + 14) == (uint32_t)reg2bin(pos, end);
+}
+
+template <class A>
+__device__ inline bool is_record_at(const A& a, int64_t n, int64_t o, int n_ref) {
+    RecHead h;
+    if (!record_head(a, n, o, n_ref, h)) return false;
     int64_t nm = o + 36;
-    if (b[nm + lrn - 1] != 0) return false;
-    for (uint32_t i = 0; i + 1 < lrn; i++) {
-        uint32_t c = b[nm + i];
+    for (uint32_t i = 0; i + 1 < h.lrn; i++) {
+        uint32_t c = a.u8(nm + i);
         if (c < 33 || c > 126 || c == '@') return false;
     }
-    int64_t cg = nm + lrn, rlen = 0;
-    for (uint32_t i = 0; i < ncig; i++) {
-        uint32_t v = ld_u32(b, cg + 4LL * i), op = v & 15;
+    int64_t cg = nm + h.lrn, rlen = 0;
+    for (uint32_t i = 0; i < h.ncig; i++) {
+        uint32_t v = a.u32(cg + 4LL * i), op = v & 15;
         if (op > 8) return false;
         if (ref_consuming(op)) rlen += v >> 4;
     }
-    if (pos >= 0) {
-        int end = pos + (int)(rlen > 0 ? rlen : 1);
-        if (ld_u16(b, o + 14) != (uint32_t)reg2bin(pos, end)) return false;
-    }
-    return true;
+    return record_bin_ok(a, o, h.pos, rlen);
+}
+
+__device__ inline bool is_record(const uint8_t* b, int64_t n, int64_t o, int n_ref) {
+    return is_record_at(GlobalBytes{b}, n, o, n_ref);
 }
 
 // ------------------------------------------------------------------ wave64 / workgroup scans

@@ -73,7 +73,7 @@ static int device_cus() {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
-        hipGetDevice(&dev);
+        (void)hipGetDevice(&dev);
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
         if (cus <= 0) cus = 256;
@@ -364,8 +364,10 @@ int cbm_record_chain(const uint8_t* data, int64_t start, int64_t n, int32_t n_re
     if (rc || w.nseg == 0) return rc;
     ARG(data, "cbm_record_chain: null data");
     unsigned g = (unsigned)((w.nseg + REC_NT - 1) / REC_NT);
-    hipLaunchKernelGGL(rec_walk, dim3(g), dim3(REC_NT), 0, st, data, start, n, n_ref, w);
-    if ((rc = check_launch("rec_walk"))) return rc;
+    const unsigned gw = (unsigned)((w.nseg + RW_WAVES - 1) / RW_WAVES);
+    hipLaunchKernelGGL(rec_walk_wave<false>, dim3(gw), dim3(RW_WAVES * CBM_WAVE), 0, st, data, start, n, (int)n_ref,
+                       SpanTab{}, w);
+    if ((rc = check_launch("rec_walk_wave"))) return rc;
     hipLaunchKernelGGL(rec_links, dim3(g), dim3(REC_NT), 0, st, start, w);
     if ((rc = check_launch("rec_links"))) return rc;
     hipLaunchKernelGGL(rec_sum, dim3(g), dim3(REC_NT), 0, st, w, result);
@@ -446,8 +448,10 @@ int cbm_record_chain_spans(const uint8_t* data, const int64_t* span_start, const
     if (n_seg > 0) {
         ARG(data, "cbm_record_chain_spans: null data");
         unsigned g = (unsigned)((n_seg + REC_NT - 1) / REC_NT);
-        hipLaunchKernelGGL(recs_walk, dim3(g), dim3(REC_NT), 0, st, data, t, w);
-        if ((rc = check_launch("recs_walk"))) return rc;
+        const unsigned gw = (unsigned)((n_seg + RW_WAVES - 1) / RW_WAVES);
+        hipLaunchKernelGGL(rec_walk_wave<true>, dim3(gw), dim3(RW_WAVES * CBM_WAVE), 0, st, data, (int64_t)0,
+                           (int64_t)0, 0, t, w);
+        if ((rc = check_launch("rec_walk_wave"))) return rc;
         hipLaunchKernelGGL(recs_links, dim3(g), dim3(REC_NT), 0, st, t, w);
         if ((rc = check_launch("recs_links"))) return rc;
         hipLaunchKernelGGL(recs_sum, dim3(g), dim3(REC_NT), 0, st, t, w, result);

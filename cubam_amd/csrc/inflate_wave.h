@@ -823,29 +823,41 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 const bool active = a < in_bits;        // thread 0 is always active
                 // pass A. Threads after the first start OVL bits before their segment, so that the
                 // walk has most likely fallen onto the true symbol boundaries when it enters the
-                // segment (then the link below finds the entry among the starts it saw at once);
-                // only starts inside the segment are remembered. A walk that stops (end of block
-                // or a bad code) before its segment starts again from the segment start.
+                // segment. A walk that stops (end of block or a bad code) before its segment starts
+                // again from the segment start. Inside the segment the walk keeps NCP checkpoints:
+                // the first symbol start at or after a + j * SEG / NCP, with the output bytes and
+                // matches counted from checkpoint 0 up to there, and in all to its exit.
                 constexpr int OVL = SEG * CBM_INFLATE_OVL_HALVES / 2;
-                uint32_t seen[SEG / 32];
+                constexpr int NCP = SEG / 32 < 8 ? SEG / 32 : 8, CPS = SEG / NCP;
+                int cp[NCP];
+                uint32_t cum[NCP];
 #pragma unroll
-                for (int k = 0; k < SEG / 32; k++) seen[k] = 0;
+                for (int j = 0; j < NCP; j++) cp[j] = 0x7fffffff, cum[j] = 0;
+                uint32_t run = 0;  // matches << 20 | bytes, from checkpoint 0
                 int p = a, stop = STOP_NONE;
                 if (active) {
                     Win w;
                     win_at(w, s, a - OVL > base ? a - OVL : base, wb);
+                    int nj = 0, nxt = a;  // the next checkpoint, and the bit it starts at
                     while (w.p < aend) {
-                        const int r = w.p - a;  // negative in the overlap: marks nothing
-#pragma unroll
-                        for (int k = 0; k < SEG / 32; k++)
-                            if ((r >> 5) == k) seen[k] |= 1u << (r & 31);
                         const int at = w.p;
+                        if (at >= nxt) {  // at most NCP times a walk
+                            do {
+#pragma unroll
+                                for (int j = 0; j < NCP; j++)
+                                    if (j == nj) cp[j] = at, cum[j] = run;
+                                nj++;
+                                nxt += CPS;
+                            } while (at >= nxt && nj < NCP);
+                            if (nj == NCP) nxt = 0x7fffffff;
+                        }
                         Sym y = win_next(w, s, wb);
                         const bool halt = y.kind == 2 || y.kind == 3 || w.p > in_bits;
                         if (halt && at < a) {
                             win_at(w, s, a, wb);
                             continue;
                         }
+                        if (nj) run += y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
                         if (y.kind == 2) { stop = STOP_EOB; break; }
                         if (halt) { stop = STOP_BAD; break; }
                     }
@@ -855,9 +867,11 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 s.stop_[tid] = active ? stop : STOP_BAD;
                 // the walk's exit is the exit from a only if it passed through a; else the first
                 // link step walks again from the predecessor's exit
-                int entry = (seen[0] & 1u) ? a : -1;
-                // a later walk that lands on a start of this first walk takes this walk's end
+                int entry = cp[0] == a ? a : -1;
+                // a later walk that lands on a checkpoint of this first walk takes this walk's end
+                // and counts from there
                 const int a_exit = p, a_stop = stop;
+                const uint32_t a_run = run;
                 __syncthreads();
                 PHASE(STAT_PASS_A);
                 // link the walks
@@ -870,31 +884,33 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     if (tid == 0) s.sv[SV_F0 + (it + 1) % 3] = 0;
                     __syncthreads();
                     bool redo = active && pstop == STOP_NONE && want != entry;
-                    int np = p, nstop = stop;
                     if (redo) {
-                        nstop = STOP_NONE;
-                        bool synced = false;
+                        int nstop = STOP_NONE, sj = -1;
+                        uint32_t r2 = 0;
                         Win w;
                         win_at(w, s, want, wb);
                         while (w.p < aend) {
-                            int r = w.p - a;
-                            if (r >= 0) {
-                                uint32_t bit = 0;
 #pragma unroll
-                                for (int k = 0; k < SEG / 32; k++)
-                                    if ((r >> 5) == k) bit = (seen[k] >> (r & 31)) & 1;
-                                if (bit) { synced = true; break; }
-                            }
+                            for (int j = 0; j < NCP; j++)
+                                if (w.p == cp[j]) sj = j;
+                            if (sj >= 0) break;
                             Sym y = win_next(w, s, wb);
+                            r2 += y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
                             if (y.kind == 2) { nstop = STOP_EOB; break; }
                             if (y.kind == 3 || w.p > in_bits) { nstop = STOP_BAD; break; }
                         }
-                        np = w.p;
-                        if (synced) {
+                        int np = w.p;
+                        if (sj >= 0) {
+                            uint32_t c = 0;
+#pragma unroll
+                            for (int j = 0; j < NCP; j++)
+                                if (j == sj) c = cum[j];
                             np = a_exit;
                             nstop = a_stop;
+                            r2 += a_run - c;
                         }
                         entry = want;
+                        run = r2;
                         if (np != p || nstop != stop) s.sv[SV_F0 + it % 3] = 1;
                         p = np;
                         stop = nstop;
@@ -914,19 +930,8 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 __syncthreads();
                 const int tend = s.sv[SV_TEND];
                 const bool live = tid <= tend && tid < NT;
-                // pass B: output bytes and matches of each live walk, entry .. exit
-                int cnt = 0, nmatch = 0;
-                if (live) {
-                    const int e = s.exit_[tid];
-                    Win w;
-                    win_at(w, s, tid == 0 ? base : s.exit_[tid - 1], wb);
-                    while (w.p < e) {
-                        Sym y = win_next(w, s, wb);
-                        if (y.kind == 0) cnt += 1;
-                        else if (y.kind == 1) cnt += y.val, nmatch++;
-                        else break;
-                    }
-                }
+                // output bytes and matches of each live walk, entry .. exit, as the walks counted them
+                const int cnt = live ? (int)(run & 0xfffff) : 0, nmatch = live ? (int)(run >> 20) : 0;
                 int64_t tot2;
                 const int64_t pre = block_exclusive_sum<NT, int64_t>(((int64_t)nmatch << 32) | (uint32_t)cnt,
                                                                       (int64_t*)s.red64, &tot2);

@@ -3,10 +3,10 @@
 // The reference tests every inflated byte offset with is_record (one thread per byte,
 // reading the name and CIGAR at each), radix-sorts the candidates and chains them
 // with pointer jumping (src/cubam/bam.py:403-418, src/cubam/_kernels.py:250-289,
-// 676-702). Here the inflated data is cut into segments of SEG bytes; each thread
-// finds the first offset of its segment that passes the structural test (confirmed by
-// the record after it), walks block_size from there over the records that start in
-// its segment, and records where it left. The walks chain when every exit is the
+// 676-702). Here the inflated data is cut into segments of REC_SEG bytes; a wavefront
+// stages its segment in LDS, finds the first offset of it that passes the structural test
+// (confirmed by the record after it), walks block_size from there over the records that
+// start in the segment, and records where it left. The walks chain when every exit is the
 // start found in the segment it lands in (segment 0 starts at the known record
 // boundary), which a second pass checks; a walk that does not chain is repaired by
 // re-walking it from the true exit of the walk before it.
@@ -15,7 +15,7 @@
 
 namespace cbm {
 
-constexpr int64_t REC_SEG = 4096;
+constexpr int64_t REC_SEG = 8192;
 constexpr int REC_NT = 256;
 
 enum { SEG_END_DATA = 1, SEG_END_INVALID = 2 };
@@ -64,12 +64,13 @@ inline int64_t rec_ws_bytes_seg(int64_t nseg) { return 8 * (8 + 9 * nseg + 1) + 
 inline int64_t rec_ws_bytes(int64_t len) { return rec_ws_bytes_seg(rec_nseg(len)); }
 
 // a record start confirmed by the record after it (or by the end of the data)
-__device__ __forceinline__ bool record_start(const uint8_t* b, int64_t n, int64_t o, int n_ref) {
-    if (!is_record(b, n, o, n_ref)) return false;
-    int64_t nx = o + 4 + (int64_t)ld_u32(b, o);
+template <class A>
+__device__ __forceinline__ bool record_start_at(const A& a, int64_t n, int64_t o, int n_ref) {
+    if (!is_record_at(a, n, o, n_ref)) return false;
+    int64_t nx = o + 4 + (int64_t)a.u32(o);
     if (nx + 36 > n) return true;
-    if (nx + 4 + (int64_t)ld_u32(b, nx) > n) return true;
-    return is_record(b, n, nx, n_ref);
+    if (nx + 4 + (int64_t)a.u32(nx) > n) return true;
+    return is_record_at(a, n, nx, n_ref);
 }
 
 struct WalkOut {
@@ -94,27 +95,6 @@ __device__ __forceinline__ WalkOut walk(const uint8_t* b, int64_t n, int64_t o, 
     if (o >= n && !w.flag) w.flag = SEG_END_DATA;
     w.e = o;
     return w;
-}
-
-__global__ void __launch_bounds__(REC_NT) rec_walk(const uint8_t* b, int64_t start, int64_t n, int n_ref, RecWs w) {
-    int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
-    if (k >= w.nseg) return;
-    int64_t lo = start + k * REC_SEG, hi = lo + REC_SEG < n ? lo + REC_SEG : n;
-    int64_t f = -1;
-    if (k == 0) f = start;
-    else
-        for (int64_t o = lo; o < hi; o++)
-            if (record_start(b, n, o, n_ref)) { f = o; break; }
-    w.f[k] = f;
-    w.pred[k] = 0;
-    WalkOut r = {-1, 0, 0, 0, 0, 0};
-    if (f >= 0) r = walk(b, n, f, hi, n_ref);
-    w.e[k] = r.e;
-    w.cnt[k] = r.cnt;
-    w.cig[k] = r.cig;
-    w.seqb[k] = r.seqb;
-    w.qual[k] = r.qual;
-    w.flag[k] = r.flag;
 }
 
 // Each walk that does not end the data must land on the start found in that segment,
@@ -219,30 +199,6 @@ __device__ __forceinline__ int64_t span_of(const SpanTab& t, int64_t k) {
     return lo;
 }
 
-__global__ void __launch_bounds__(REC_NT) recs_walk(const uint8_t* b, SpanTab t, RecWs w) {
-    int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
-    if (k >= w.nseg) return;
-    int64_t s = span_of(t, k);
-    int64_t s0 = t.start[s], n = t.end[s], k0 = t.segbase[s];
-    int n_ref = t.n_ref[s];
-    int64_t lo = s0 + (k - k0) * REC_SEG, hi = lo + REC_SEG < n ? lo + REC_SEG : n;
-    int64_t f = -1;
-    if (k == k0) f = s0;
-    else
-        for (int64_t o = lo; o < hi; o++)
-            if (record_start(b, n, o, n_ref)) { f = o; break; }
-    w.f[k] = f;
-    w.pred[k] = 0;
-    WalkOut r = {-1, 0, 0, 0, 0, 0};
-    if (f >= 0) r = walk(b, n, f, hi, n_ref);
-    w.e[k] = r.e;
-    w.cnt[k] = r.cnt;
-    w.cig[k] = r.cig;
-    w.seqb[k] = r.seqb;
-    w.qual[k] = r.qual;
-    w.flag[k] = r.flag;
-}
-
 __global__ void __launch_bounds__(REC_NT) recs_links(SpanTab t, RecWs w) {
     int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
     int64_t bad = 0;
@@ -292,6 +248,137 @@ __global__ void __launch_bounds__(REC_NT) recs_sum(SpanTab t, RecWs w, int64_t* 
 __global__ void recs_bases(SpanTab t, RecWs w, int64_t* span_rec) {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s <= t.n; s += (int64_t)gridDim.x * blockDim.x)
         span_rec[s] = w.base[t.segbase[s]];
+}
+
+// ------------------------------------------------------------------ the walk, a wavefront a segment
+// Each wavefront stages its segment and the first RW_OVER bytes after it in LDS (16-byte loads),
+// then finds the segment's first record start with a lane per candidate offset, and walks the
+// records with every lane: the fixed fields are read alike by all lanes, the read name and the
+// CIGAR ops a byte or an op a lane. Bytes past the staged window are read from global memory.
+constexpr int RW_WAVES = 4;
+constexpr int RW_OVER = 512;
+constexpr int RW_STAGE = (int)REC_SEG + RW_OVER + 16;
+
+struct StagedBytes {
+    const uint8_t* g;  // the data
+    const uint8_t* l;  // LDS copy of g[base, base + lim)
+    int64_t base, lim;
+    __device__ __forceinline__ uint32_t u8(int64_t o) const {
+        const int64_t r = o - base;
+        return (uint64_t)r < (uint64_t)lim ? l[r] : g[o];
+    }
+    __device__ __forceinline__ uint32_t u16(int64_t o) const { return u8(o) | (u8(o + 1) << 8); }
+    __device__ __forceinline__ uint32_t u32(int64_t o) const {
+        return u8(o) | (u8(o + 1) << 8) | (u8(o + 2) << 16) | (u8(o + 3) << 24);
+    }
+};
+
+// walk() with the wavefront: records that start in [o, hi) from o, a record start. The chain
+// through block_size is followed 64 records at a time (the same by every lane; lane j keeps the
+// j-th offset), then each lane tests its record; the first that fails ends the walk there.
+template <class A>
+__device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane) {
+    WalkOut w = {o, 0, 0, 0, 0, 0};
+    while (o < hi && !w.flag) {
+        int64_t mine = -1;
+        int nrec = 0;
+        while (nrec < CBM_WAVE && o < hi) {
+            if (o + 4 > n) { w.flag = SEG_END_DATA; break; }
+            const int64_t bs = (int64_t)a.u32(o);
+            if (o + 4 + bs > n) { w.flag = SEG_END_DATA; break; }
+            if (lane == nrec) mine = o;
+            nrec++;
+            o += 4 + bs;
+        }
+        const bool bad = mine >= 0 && !is_record_at(a, n, mine, n_ref);
+        const uint64_t badm = __ballot(bad);
+        const int good = badm ? __ffsll((unsigned long long)badm) - 1 : nrec;
+        int64_t cig = 0, seqb = 0, qual = 0;
+        if (lane < good) {
+            const uint32_t lseq = a.u32(mine + 20);
+            cig = a.u16(mine + 16);
+            seqb = (lseq + 1) / 2;
+            qual = lseq;
+        }
+        w.cnt += good;
+        w.cig += wave_sum(cig);
+        w.seqb += wave_sum(seqb);
+        w.qual += wave_sum(qual);
+        if (badm) {
+            o = __shfl(mine, good, CBM_WAVE);
+            w.flag = SEG_END_INVALID;
+        }
+    }
+    if (o >= n && !w.flag) w.flag = SEG_END_DATA;
+    w.e = o;
+    return w;
+}
+
+// Segment k of one chunk (SPANS false: the data [start, n), n_ref references) or of a batch of
+// spans (SPANS true: span table t): its first record start f (the span or chunk start for its
+// first segment) and the walk from f over the records that start in the segment.
+template <bool SPANS>
+__global__ void __launch_bounds__(RW_WAVES * CBM_WAVE)
+    rec_walk_wave(const uint8_t* b, int64_t start, int64_t n_all, int n_ref_all, SpanTab t, RecWs w) {
+    __shared__ __align__(16) uint8_t stg[RW_WAVES][RW_STAGE];
+    const int wv = threadIdx.x / CBM_WAVE, lane = threadIdx.x % CBM_WAVE;
+    const int64_t k = (int64_t)blockIdx.x * RW_WAVES + wv;
+    const bool have = k < w.nseg;  // wave-uniform
+    int64_t lo = 0, hi = 0, n = 0, first = 0;
+    int n_ref = n_ref_all;
+    if (have) {
+        if constexpr (SPANS) {
+            const int64_t s = span_of(t, k);
+            const int64_t k0 = t.segbase[s];
+            n = t.end[s];
+            n_ref = t.n_ref[s];
+            lo = t.start[s] + (k - k0) * REC_SEG;
+            first = k == k0;
+        } else {
+            n = n_all;
+            lo = start + k * REC_SEG;
+            first = k == 0;
+        }
+        hi = lo + REC_SEG < n ? lo + REC_SEG : n;
+    }
+    // the window starts at the 16-byte boundary at or before lo (at lo itself, byte by byte, if that
+    // boundary lies before the data)
+    int64_t abase = lo - (int64_t)((uintptr_t)(b + lo) & 15);
+    const bool vec = abase >= 0;
+    if (!vec) abase = lo;
+    const int64_t lim = have ? (n - abase < RW_STAGE ? n - abase : RW_STAGE) : 0;
+    uint8_t* l = stg[wv];
+    const int nvec = vec ? (int)(lim / 16) : 0;
+    for (int i = lane; i < nvec; i += CBM_WAVE) ((uint4*)l)[i] = *(const uint4*)(b + abase + 16LL * i);
+    for (int i = nvec * 16 + lane; i < lim; i += CBM_WAVE) l[i] = b[abase + i];
+    __syncthreads();
+    if (!have) return;
+    const StagedBytes a{b, l, abase, lim};
+    int64_t f = -1;
+    if (first) {
+        f = lo;
+    } else {
+        for (int64_t o0 = lo; o0 < hi; o0 += CBM_WAVE) {
+            const int64_t o = o0 + lane;
+            const uint64_t m = __ballot(o < hi && record_start_at(a, n, o, n_ref));
+            if (m) {
+                f = o0 + __ffsll((unsigned long long)m) - 1;
+                break;
+            }
+        }
+    }
+    WalkOut r = {-1, 0, 0, 0, 0, 0};
+    if (f >= 0) r = walk_wave(a, n, f, hi, n_ref, lane);
+    if (lane == 0) {
+        w.f[k] = f;
+        w.pred[k] = 0;
+        w.e[k] = r.e;
+        w.cnt[k] = r.cnt;
+        w.cig[k] = r.cig;
+        w.seqb[k] = r.seqb;
+        w.qual[k] = r.qual;
+        w.flag[k] = r.flag;
+    }
 }
 
 // ------------------------------------------------------------------ kernel 4: scatter

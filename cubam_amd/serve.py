@@ -48,7 +48,13 @@ from cubam_amd.bam import RecordBatch
 from cubam_amd.index import BlockIndex, Span, build_index, host_bytes
 
 _U8, _I32, _I64 = torch.uint8, torch.int32, torch.int64
-SEG_BYTES = 4096  # segment of the record chain (REC_SEG of the kernels)
+
+
+def segment_bytes() -> int:
+    """The record chain's segment size, as the library reports it."""
+    seg = ctypes.c_int64(0)
+    _lib.load().cbm_record_chain_workspace(0, ctypes.byref(seg))
+    return int(seg.value)
 
 
 class BamSource:
@@ -144,7 +150,8 @@ def decode_spans(items: list[tuple[BamSource, Span]], tags=("CB", "UB"), ws: _de
     s_start = np.array([sp.start - sp.u0 for _, sp in items], np.int64) + base
     s_end = np.array([sp.end - sp.u0 for _, sp in items], np.int64) + base
     seg_base = np.zeros(n + 1, np.int64)
-    np.cumsum((s_end - s_start + SEG_BYTES - 1) // SEG_BYTES, out=seg_base[1:])
+    seg_bytes = segment_bytes()
+    np.cumsum((s_end - s_start + seg_bytes - 1) // seg_bytes, out=seg_base[1:])
     n_seg = int(seg_base[-1])
     nref = np.array([src.n_ref for src, _ in items], np.int32)
     total_out = int(out_off[-1])
