@@ -8,14 +8,15 @@
 // Huffman block gives thread t the SEG bits that start at base + t * SEG.
 // Pass A decodes speculatively, from SEG bits before the segment (so that the walk
 // is most likely on the true boundaries when it enters it) to the first symbol
-// that starts at or after the next segment, and remembers every symbol start it
-// saw inside its segment. The walks are then linked: thread t's true entry is
-// thread t-1's exit; a thread whose entry changed decodes again from it until it
-// lands on a start it already saw (from there its first walk was right) -
-// repeated until nothing changes.
-// Pass B counts each thread's output bytes and matches, a workgroup prefix sum
-// places them, and pass C writes the literals into the LDS output window and the
-// matches, in output order, into a per-workgroup list in global scratch. Pass D
+// that starts at or after the next segment, and keeps checkpoints inside its
+// segment: the first symbol start at or after every SEG / NCP-th bit, with the
+// output bytes and matches counted up to there. The walks are then linked: thread
+// t's true entry is thread t-1's exit; a thread whose entry changed decodes again
+// from it until it lands on a checkpoint (from there its first walk was right, and
+// its count from there is known) - repeated until nothing changes. A workgroup
+// prefix sum of the walks' counts places them, and pass C writes the literals into
+// the LDS output window and the matches, in output order, into a per-workgroup
+// list in global scratch. Pass D
 // applies the matches. Variants whose staging area holds a bit per output byte
 // keep the round's pending bytes in a bitmap there and apply, step by step, every
 // match whose source bytes are no longer pending; the others go frontier by
