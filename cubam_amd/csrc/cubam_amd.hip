@@ -184,7 +184,7 @@ static int launch_wave(const uint8_t* src, const int64_t* blk_off, const int32_t
     if (g > inflate_max_groups()) g = inflate_max_groups();  // the workspace holds that many match lists
     if (g > n_blocks) g = n_blocks;
     if (g < 1) g = 1;
-    uint64_t* mlists = (uint64_t*)((uint8_t*)counter + 256);
+    uint64_t* mlists = (uint64_t*)((uint8_t*)counter + INFLATE_WS_HDR);
     hipLaunchKernelGGL((inflate_wave<NT, SEG>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
                        crc_expect, out_off, n_blocks, dst, crc, status, result, (unsigned long long*)counter, mlists,
                        (unsigned long long*)counter + 8);
@@ -273,7 +273,7 @@ int cbm_bgzf_scan(const uint8_t* buf, int64_t n, int eof, int64_t cap, int64_t* 
 // ------------------------------------------------------------------ kernel 2
 int64_t cbm_inflate_workspace(int64_t n_blocks) {
     (void)n_blocks;
-    return 256 + inflate_max_groups() * (int64_t)MLIST_CAP * 8;
+    return INFLATE_WS_HDR + inflate_max_groups() * (int64_t)MLIST_CAP * 8;
 }
 
 int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,

@@ -91,7 +91,10 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 #endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
        STAT_DSTEPS, STAT_LINKIT, STAT_H_STAGE, STAT_H_PARSE, STAT_H_CL, STAT_H_CLDEC, STAT_BLOCKEND, STAT_H_STAGE2,
-       STAT_CLWIN, STAT_DYN, STAT_D_MARK, STAT_D_STEP0, STAT_MBIG, STAT_MSUM, STAT_RSUM, STAT_N };
+       STAT_CLWIN, STAT_DYN, STAT_D_MARK, STAT_D_STEP0, STAT_MBIG, STAT_MSUM, STAT_RSUM, STAT_D_IDLE, STAT_D_LANEIT, STAT_D_WAVEM, STAT_N };
+// the inflate workspace: the block counter, 7 spare words, the statistics, then the match lists
+constexpr int INFLATE_WS_HDR = 512;
+static_assert(8 * (8 + STAT_N) <= INFLATE_WS_HDR, "statistics overrun the workspace header");
 
 // matches of a round, per workgroup, in global scratch: a block of at most 65,536 bytes has at
 // most 21,846 matches of 3 bytes or more
@@ -1076,6 +1079,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                 if ((left & (1u << u)) && match_ready(pend, mm[u], op)) rdy |= 1u << u;
                             const uint64_t lanes = __ballot(rdy != 0);
                             if (!lanes) {
+                                COUNT(STAT_D_IDLE);
                                 if (++idle > D_IDLE_MAX) {
                                     if ((tid % CBM_WAVE) == 0) {
                                         atomicOr(&s.sv[SV_ERR], CBM_BLOCK_DEFLATE);
@@ -1089,6 +1093,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             left &= ~rdy;
                             __threadfence_block();  // read the sources after their bits were seen clear
                             if (__popcll(lanes) > D_LANE_COPY) {
+                                COUNT(STAT_D_LANEIT);
 #pragma unroll
                                 for (int u = 0; u < MD_K; u++)
                                     if (rdy & (1u << u))
@@ -1101,6 +1106,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                     while (bal) {
                                         const int from = __ffsll((unsigned long long)bal) - 1;
                                         bal &= bal - 1;
+                                        COUNT(STAT_D_WAVEM);
                                         wave_copy_match(s.out, __shfl(mm[u], from, CBM_WAVE), tid % CBM_WAVE);
                                     }
                                 }

@@ -18,6 +18,9 @@ so no block scan and no host round trip is needed to find the blocks.
   record chain, to size the columns.
 * **Overlap.** Each worker has its own HIP stream and workspace, so requests
   that arrive while a batch is on the GPU form the next batch and run beside it.
+  While no worker is idle, a worker takes its next batch as soon as the current
+  one is launched and copies it to the device on a second stream
+  (:func:`stage_spans`) while the current one decodes (:func:`decode_staged`).
 * **Residency.** Decoded results stay on the device until evicted or released.
 
 ``mode="plain"`` serves the same requests one per call on one stream with the
@@ -76,12 +79,17 @@ class BamSource:
 
 # ------------------------------------------------------------------ batched decode
 class _Pinned:
-    """A grow-only pinned host buffer."""
+    """A grow-only pinned host buffer. ``event``, when set, marks the end of the last copy out of
+    it; :meth:`get` waits for it before the buffer is written again."""
 
     def __init__(self):
         self.t = torch.empty(0, dtype=_U8)
+        self.event: torch.cuda.Event | None = None
 
     def get(self, n: int) -> torch.Tensor:
+        if self.event is not None:
+            self.event.synchronize()
+            self.event = None
         if self.t.numel() < n:
             self.t = torch.empty(max(n, 2 * self.t.numel(), 1 << 16), dtype=_U8, pin_memory=True)
         return self.t[:n]
@@ -106,23 +114,28 @@ def result_bytes(rb: RecordBatch) -> int:
     return total
 
 
-def decode_spans(items: list[tuple[BamSource, Span]], tags=("CB", "UB"), ws: _decode.Workspace | None = None,
-                 pinned: _Pinned | None = None) -> list[RecordBatch]:
-    """Decode the records of many spans (of one or more files) in one batch, on the current stream.
+@dataclass
+class Staged:
+    """A batch whose tables and compressed bytes are on their way to the device (see
+    :func:`stage_spans`); :func:`decode_staged` decodes it."""
 
-    Returns one :class:`RecordBatch` per span, in order. They share the batch's
-    device buffers: ``offset`` points into the shared ``data`` and the CSR offsets
-    (``cigar_off``, ``seq_off``, ``qual_off``) into the shared CSR arrays. Raises
-    on corrupt data as the per-chunk path does.
-    """
-    L = _lib.load()
-    ws = ws or _decode.default_workspace()
+    items: list
+    plan: SimpleNamespace  # host-side sizes of the batch
+    tab: torch.Tensor | None  # block and span tables (device)
+    staging: torch.Tensor | None  # compressed bytes of every span, side by side (device)
+    event: torch.cuda.Event | None  # recorded after the copies
+
+
+def stage_spans(items: list[tuple[BamSource, Span]], pinned: _Pinned | None = None,
+                stream: torch.cuda.Stream | None = None) -> Staged:
+    """The host plan of a batch and its host-to-device copies (one packed table, then the
+    compressed bytes of each span), issued on ``stream`` (the current stream by default), so
+    that a batch can be copied while the one before it decodes."""
     pinned = pinned or _Pinned()
     dev = _decode.device()
-    st = stream_ptr()
     n = len(items)
     if n == 0:
-        return []
+        return Staged(items, SimpleNamespace(n=0), None, None, None)
     # ---- host plan: block table, span table, staging layout
     nbs = np.array([sp.k1 - sp.k0 for _, sp in items], np.int64)
     NB = int(nbs.sum())
@@ -172,18 +185,56 @@ def decode_spans(items: list[tuple[BamSource, Span]], tags=("CB", "UB"), ws: _de
     for p in parts32:
         h32[o : o + p.size] = p
         o += p.size
-    tab = torch.empty(nbytes, dtype=_U8, device=dev)
-    tab.copy_(host, non_blocking=True)
+    stream = stream or torch.cuda.current_stream()
+    with torch.cuda.stream(stream):
+        tab = torch.empty(nbytes, dtype=_U8, device=dev)
+        tab.copy_(host, non_blocking=True)
+        # ---- compressed bytes of every span, side by side
+        staging = torch.empty(int(cpos[-1]) + PAD, dtype=_U8, device=dev)
+        for r, (src, sp) in enumerate(items):
+            staging[cpos[r] : cpos[r] + clen[r]].copy_(src.host[sp.c0 : sp.c1], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+    pinned.event = ev
+    plan = SimpleNamespace(n=n, NB=NB, n64=n64, n32=n32, n_seg=n_seg, total_out=total_out,
+                           cuts64=np.cumsum([0] + [p.size for p in parts64]),
+                           cuts32=np.cumsum([0] + [p.size for p in parts32]))
+    return Staged(items, plan, tab, staging, ev)
+
+
+def decode_spans(items: list[tuple[BamSource, Span]], tags=("CB", "UB"), ws: _decode.Workspace | None = None,
+                 pinned: _Pinned | None = None) -> list[RecordBatch]:
+    """Decode the records of many spans (of one or more files) in one batch, on the current stream.
+
+    Returns one :class:`RecordBatch` per span, in order. They share the batch's
+    device buffers: ``offset`` points into the shared ``data`` and the CSR offsets
+    (``cigar_off``, ``seq_off``, ``qual_off``) into the shared CSR arrays. Raises
+    on corrupt data as the per-chunk path does.
+    """
+    return decode_staged(stage_spans(items, pinned), tags, ws)
+
+
+def decode_staged(staged: Staged, tags=("CB", "UB"), ws: _decode.Workspace | None = None) -> list[RecordBatch]:
+    """Decode a staged batch on the current stream (after its copies): the kernels of
+    :func:`decode_spans`."""
+    L = _lib.load()
+    ws = ws or _decode.default_workspace()
+    dev = _decode.device()
+    st = stream_ptr()
+    items, pl = staged.items, staged.plan
+    n = pl.n
+    if n == 0:
+        return []
+    cur = torch.cuda.current_stream()
+    cur.wait_event(staged.event)
+    tab, staging = staged.tab, staged.staging
+    tab.record_stream(cur)
+    staging.record_stream(cur)
+    NB, n_seg, total_out, n64, cuts64, cuts32 = pl.NB, pl.n_seg, pl.total_out, pl.n64, pl.cuts64, pl.cuts32
     d64 = tab[: 8 * n64].view(_I64)
     d32 = tab[8 * n64 :].view(_I32)
-    cuts64 = np.cumsum([0] + [p.size for p in parts64])
-    cuts32 = np.cumsum([0] + [p.size for p in parts32])
     blk_off_d, out_off_d, ss_d, se_d, sb_d = (d64[cuts64[i] : cuts64[i + 1]] for i in range(5))
     bsize_d, isize_d, crc_d, nref_d = (d32[cuts32[i] : cuts32[i + 1]] for i in range(4))
-    # ---- compressed bytes of every span, side by side
-    staging = torch.empty(int(cpos[-1]) + PAD, dtype=_U8, device=dev)
-    for r, (src, sp) in enumerate(items):
-        staging[cpos[r] : cpos[r] + clen[r]].copy_(src.host[sp.c0 : sp.c1], non_blocking=True)
     # ---- kernel 2: every block of the batch in one launch
     data = torch.empty(total_out + PAD, dtype=_U8, device=dev)
     crc_out = torch.empty(max(NB, 1), dtype=torch.uint32, device=dev)
@@ -207,6 +258,7 @@ def decode_spans(items: list[tuple[BamSource, Span]], tags=("CB", "UB"), ws: _de
     h[9 : 10 + n].copy_(span_rec, non_blocking=True)
     torch.cuda.current_stream().synchronize()
     del staging
+    staged.staging = None  # the compressed bytes are no longer needed
     if int(h[8]):
         _decode.raise_block_errors(SimpleNamespace(status=status[:NB]), int(h[8]))
     if int(h[_lib.REC_BAD_LINKS]):
@@ -339,6 +391,7 @@ class Server:
         self.in_flight_bytes = 0
         self.peak_bytes = 0
         self._closing = False
+        self._waiting = 0  # workers waiting for requests
         self._ids = 0
         self.batches: list[int] = []
         n_workers = 1 if mode == "plain" else max(1, workers)
@@ -401,7 +454,9 @@ class Server:
             self.resident_bytes -= nb
         return self.in_flight_bytes + self.resident_bytes + need <= self.budget
 
-    def _take(self) -> list[Request] | None:
+    def _take(self, block: bool = True) -> list[Request] | None:
+        """The next batch of admitted requests; None when the server closes (or, with block
+        False, when nothing can be admitted now)."""
         with self._cv:
             while True:
                 if self._pending:
@@ -424,20 +479,45 @@ class Server:
                             r.admitted = now
                             r.batch_size = len(batch)
                         return batch
-                if self._closing and not self._pending:
+                if (self._closing and not self._pending) or not block:
                     return None
+                self._waiting += 1
                 self._cv.wait()
+                self._waiting -= 1
+
+    def _stage(self, batch: list[Request], pinned: _Pinned, stream) -> Staged | BaseException:
+        try:
+            return stage_spans([(self.sources[r.file], r.span) for r in batch], pinned, stream)
+        except Exception as e:  # noqa: BLE001
+            return e
 
     def _work(self, i: int) -> None:
+        """A worker: decodes one batch at a time on its stream. In served mode it takes the next
+        batch as soon as the current one is launched and copies it on a second stream while the
+        current one decodes."""
         stream = torch.cuda.Stream()
+        copy = torch.cuda.Stream() if self.mode != "plain" else None
         ws = _decode.Workspace()
-        pinned = _Pinned()
+        pins = [_Pinned(), _Pinned()]
+        slot = 0
+        nxt = None
         with torch.cuda.stream(stream):
             while True:
-                batch = self._take()
-                if batch is None:
-                    return
-                results = self._run(batch, ws, pinned)
+                if nxt is None:
+                    batch = self._take()
+                    if batch is None:
+                        return
+                    staged = self._stage(batch, pins[slot], copy) if copy is not None else None
+                    slot ^= 1
+                else:
+                    batch, staged = nxt
+                    nxt = None
+                if copy is not None and not self._waiting:  # no worker is idle
+                    nb = self._take(block=False)
+                    if nb is not None:
+                        nxt = (nb, self._stage(nb, pins[slot], copy))
+                        slot ^= 1
+                results = self._run(batch, ws, pins[slot], staged)
                 now = time.perf_counter()
                 with self._cv:
                     self.in_flight_bytes -= sum(r.est for r in batch)
@@ -456,13 +536,15 @@ class Server:
                         r.done.set()
                     self._cv.notify_all()
 
-    def _run(self, batch: list[Request], ws, pinned) -> list:
+    def _run(self, batch: list[Request], ws, pinned, staged) -> list:
         try:
             if self.mode == "plain":
                 r = batch[0]
                 out = [decode_plain(self.sources[r.file], r.span, self.tags, ws)]
             else:
-                out = decode_spans([(self.sources[r.file], r.span) for r in batch], self.tags, ws, pinned)
+                if isinstance(staged, BaseException):
+                    raise staged
+                out = decode_staged(staged, self.tags, ws)
             for k, r in enumerate(batch):
                 if r.cells is not None:
                     out[k] = write_back(out[k], r.cells, ws=ws)
