@@ -383,9 +383,24 @@ __device__ __forceinline__ void copy_bytes(uint8_t* out, int d, int s, int n) {
         const uint32_t* w = (const uint32_t*)out;
         uint32_t* wd = (uint32_t*)out;
         const int sh = s & 3;
+        // 16 bytes a step: five loads, then four stores, so one LDS round trip per step (the
+        // source never overlaps the destination here)
+        for (; n >= 16; n -= 16, d += 16, s += 16) {
+            const int sw = s >> 2;
+            const uint32_t a0 = w[sw], a1 = w[sw + 1], a2 = w[sw + 2], a3 = w[sw + 3], a4 = w[sw + 4];
+            const int dw = d >> 2;
+            if (sh == 0) {
+                wd[dw] = a0, wd[dw + 1] = a1, wd[dw + 2] = a2, wd[dw + 3] = a3;
+            } else {
+                wd[dw] = __builtin_amdgcn_alignbyte(a1, a0, (uint32_t)sh);
+                wd[dw + 1] = __builtin_amdgcn_alignbyte(a2, a1, (uint32_t)sh);
+                wd[dw + 2] = __builtin_amdgcn_alignbyte(a3, a2, (uint32_t)sh);
+                wd[dw + 3] = __builtin_amdgcn_alignbyte(a4, a3, (uint32_t)sh);
+            }
+        }
         if (sh == 0) {
             for (; n >= 4; n -= 4, d += 4, s += 4) wd[d >> 2] = w[s >> 2];
-        } else {
+        } else if (n >= 4) {
             uint32_t lo = w[s >> 2];
             for (; n >= 4; n -= 4, d += 4, s += 4) {
                 const uint32_t hi = w[(s >> 2) + 1];
@@ -454,6 +469,13 @@ __device__ __forceinline__ bool match_ready(const uint32_t* pend, uint64_t m, in
     const int len = (int)((m >> 33) & 0x1ff), src = dest - dist;
     const int lo = src < op ? op : src, hi = src + (dist < len ? dist : len);
     return !(lo < hi && any_bits(pend, lo, hi));
+}
+
+// lane `from`'s x (from is wavefront-uniform): two readlanes, no LDS round trip
+__device__ __forceinline__ uint64_t lane_u64(uint64_t x, int from) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, from);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), from);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Match m copied by the lanes of a wavefront, a byte a lane: the bytes read, [src, src + len)
@@ -887,8 +909,9 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     int pstop = tid == 0 ? STOP_NONE : s.stop_[tid - 1];
                     if (tid == 0) s.sv[SV_F0 + (it + 1) % 3] = 0;
                     __syncthreads();
-                    bool redo = active && pstop == STOP_NONE && want != entry;
+                    const bool redo = active && pstop == STOP_NONE && want != entry;
                     if (redo) {
+                        // walk from the new entry until a checkpoint of the first walk
                         int nstop = STOP_NONE, sj = -1;
                         uint32_t r2 = 0;
                         Win w;
@@ -1107,7 +1130,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                         const int from = __ffsll((unsigned long long)bal) - 1;
                                         bal &= bal - 1;
                                         COUNT(STAT_D_WAVEM);
-                                        wave_copy_match(s.out, __shfl(mm[u], from, CBM_WAVE), tid % CBM_WAVE);
+                                        wave_copy_match(s.out, lane_u64(mm[u], from), tid % CBM_WAVE);
                                     }
                                 }
                             }
