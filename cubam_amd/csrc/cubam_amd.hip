@@ -179,7 +179,8 @@ static int launch_wave(const uint8_t* src, const int64_t* blk_off, const int32_t
                             (int)lds);
         attr = true;
     }
-    const size_t per_cu = (160 * 1024) / (lds + 1024);
+    // workgroups that fit a CU's 160 KiB of LDS (allocated in 512-byte granules)
+    const size_t per_cu = (160 * 1024) / ((lds + 511) / 512 * 512);
     int64_t g = groups > 0 ? groups : (int64_t)device_cus() * (int64_t)(per_cu ? per_cu : 1);
     if (g > inflate_max_groups()) g = inflate_max_groups();  // the workspace holds that many match lists
     if (g > n_blocks) g = n_blocks;
