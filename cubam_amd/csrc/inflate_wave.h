@@ -1216,13 +1216,35 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
         if (err == CBM_BLOCK_OK && op != cap) err = CBM_BLOCK_SIZE, site = SITE_SIZE;
         const int nout = err == CBM_BLOCK_OK ? cap : (op < cap ? op : cap);
         uint8_t* d = dst + out_off[b];
-        for (int i = tid; i < nout; i += NT) d[i] = s.out[i];
+        {
+            // the output in 16-byte stores where d + i is 16-byte aligned (a head and a tail of
+            // single bytes), each read from LDS as five words realigned
+            const int head = (int)((16 - ((uintptr_t)d & 15)) & 15) < nout ? (int)((16 - ((uintptr_t)d & 15)) & 15) : nout;
+            if (tid < head) d[tid] = s.out[tid];
+            const int nvec = (nout - head) / 16;
+            const uint32_t* ow = (const uint32_t*)s.out;
+            for (int v = tid; v < nvec; v += NT) {
+                const int i = head + 16 * v, w0 = i >> 2;
+                const uint32_t sh = (uint32_t)(i & 3);
+                const uint32_t x0 = ow[w0], x1 = ow[w0 + 1], x2 = ow[w0 + 2], x3 = ow[w0 + 3], x4 = ow[w0 + 4];
+                uint4 q;
+                q.x = __builtin_amdgcn_alignbyte(x1, x0, sh);
+                q.y = __builtin_amdgcn_alignbyte(x2, x1, sh);
+                q.z = __builtin_amdgcn_alignbyte(x3, x2, sh);
+                q.w = __builtin_amdgcn_alignbyte(x4, x3, sh);
+                *(uint4*)(d + i) = q;
+            }
+            for (int i = head + 16 * nvec + tid; i < nout; i += NT) d[i] = s.out[i];
+        }
         // CRC32 of the output: slice tid covers [nout - SLICE*(NT - tid), nout - SLICE*(NT-1-tid)).
-        // The byte table goes in the staging area, free once the last round is done.
+        // The tables go in the staging area, free once the last round is done: with room for
+        // four, slicing-by-4 (Kounavis and Berry, 2008: table k maps a byte followed by k zero
+        // bytes), four bytes a step; else a byte a step.
+        constexpr int CRC_TABS = S::IN_WORDS >= 1024 ? 4 : 1;
         uint32_t* crcT = s.in;
-        for (int i = tid; i < 256; i += NT) {
-            uint32_t t = i;
-            for (int k = 0; k < 8; k++) t = (t & 1) ? 0xEDB88320u ^ (t >> 1) : t >> 1;
+        for (int i = tid; i < 256 * CRC_TABS; i += NT) {
+            uint32_t t = i & 255;
+            for (int k = 0; k < 8 * (1 + (i >> 8)); k++) t = (t & 1) ? 0xEDB88320u ^ (t >> 1) : t >> 1;
             crcT[i] = t;
         }
         __syncthreads();
@@ -1232,7 +1254,16 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             if (s2 < 0) s2 = 0;
             if (e2 > s2) {
                 uint32_t x = 0xFFFFFFFFu;
-                for (int k = s2; k < e2; k++) x = crcT[(x ^ s.out[k]) & 255] ^ (x >> 8);
+                int k = s2;
+                if constexpr (CRC_TABS == 4) {
+                    for (; k + 4 <= e2; k += 4) {
+                        x ^= (uint32_t)s.out[k] | ((uint32_t)s.out[k + 1] << 8) | ((uint32_t)s.out[k + 2] << 16) |
+                             ((uint32_t)s.out[k + 3] << 24);
+                        x = crcT[768 + (x & 255)] ^ crcT[512 + ((x >> 8) & 255)] ^ crcT[256 + ((x >> 16) & 255)] ^
+                            crcT[x >> 24];
+                    }
+                }
+                for (; k < e2; k++) x = crcT[(x ^ s.out[k]) & 255] ^ (x >> 8);
                 c = crc_mulmod(xfac, x ^ 0xFFFFFFFFu);
             }
         }
