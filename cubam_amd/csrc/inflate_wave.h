@@ -150,9 +150,11 @@ struct ClSym {
 };
 constexpr int CL_SEG = 32;  // bits a lane takes: more than the 14 of the longest symbol
 
+// the symbol at bit r of a lane's window (win holds 64 staged bits from the lane's first bit; a
+// symbol starts at r < CL_SEG and takes at most 14 bits)
 template <class S>
-__device__ __forceinline__ ClSym cl_sym(const S& s, int p, int wbase) {
-    const uint32_t v = peek32(s, p, wbase);
+__device__ __forceinline__ ClSym cl_sym(const S& s, uint64_t win, int r) {
+    const uint32_t v = (uint32_t)(win >> r);
     const uint32_t e = s.dist[v & 127];
     const int l = e & 15, sym = (e >> 4) & 0x1ff;
     const uint32_t x = v >> l;
@@ -691,11 +693,12 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             break;
                         }
                         const int a = wbase + lane * CL_SEG, aend = a + CL_SEG;
+                        const uint64_t win = peek64(s, a, wb1);  // every symbol this lane decodes
                         uint32_t seen = 0;
                         int p = a, bad = 0;
                         while (p < aend) {
                             seen |= 1u << (p - a);
-                            const ClSym c = cl_sym(s, p, wb1);
+                            const ClSym c = cl_sym(s, win, p - a);
                             if (!c.bits || p + c.bits > in_bits) {
                                 bad = 1;
                                 break;
@@ -718,7 +721,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                         nb = a_bad;
                                         break;
                                     }
-                                    const ClSym c = cl_sym(s, q, wb1);
+                                    const ClSym c = cl_sym(s, win, q - a);
                                     if (!c.bits || q + c.bits > in_bits) {
                                         nb = 1;
                                         break;
@@ -730,6 +733,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                 bad = nb;
                                 entry = want;
                             }
+                            COUNT(STAT_RSUM);  // code-length link iterations
                             if (!__ballot(changed)) break;
                         }
                         // lanes up to the first that stopped on a bad symbol hold the true walk
@@ -739,7 +743,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         int cnt = 0, last = -1;
                         if (alive) {
                             for (int q = entry; q < p;) {
-                                const ClSym c = cl_sym(s, q, wb1);
+                                const ClSym c = cl_sym(s, win, q - a);
                                 if (!c.bits) break;  // not met: the symbols before p are valid
                                 cnt += c.rep;
                                 last = c.sym < 16 ? c.sym : c.sym == 16 ? last : 0;
@@ -759,7 +763,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         if (alive && n < total) {
                             int q = entry;
                             while (q < p && n < total) {
-                                const ClSym c = cl_sym(s, q, wb1);
+                                const ClSym c = cl_sym(s, win, q - a);
                                 int val = 0;
                                 if (!c.bits) {
                                     myerr = 1;
