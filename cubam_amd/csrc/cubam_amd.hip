@@ -627,11 +627,11 @@ int cbm_scatter(const uint8_t* data, const int64_t* offsets, int64_t n_rec, cons
     int32_t* ncig = (int32_t*)ws;
     int32_t* lseq = ncig + n_rec;
     void* scan_ws = (uint8_t*)ws + ((8 * n_rec + 255) / 256) * 256;
-    if (n_rec > 0) {
-        ARG(data && offsets, "cbm_scatter: null pointer");
-        unsigned g = (unsigned)((n_rec + FIELDS_NT - 1) / FIELDS_NT);
-        hipLaunchKernelGGL(rec_fields, dim3(g), dim3(FIELDS_THREADS), 0, st, data, offsets, n_rec, c, ncig, lseq);
-        if ((rc = check_launch("rec_fields"))) return rc;
+    if (n_rec > 0) ARG(data && offsets, "cbm_scatter: null pointer");
+    if (n_rec > 0 && (c.cigar_off || c.seq_off || c.qual_off)) {
+        unsigned g = (unsigned)((n_rec + 255) / 256);
+        hipLaunchKernelGGL(rec_sizes, dim3(g), dim3(256), 0, st, data, offsets, n_rec, ncig, lseq);
+        if ((rc = check_launch("rec_sizes"))) return rc;
     }
     if (c.cigar_off &&
         (rc = exclusive_scan(RecNcig{ncig}, StoreI64{c.cigar_off}, (const int64_t*)nullptr, n_rec, scan_ws,
@@ -645,10 +645,10 @@ int cbm_scatter(const uint8_t* data, const int64_t* offsets, int64_t n_rec, cons
         (rc = exclusive_scan(RecQual{lseq}, StoreI64{c.qual_off}, (const int64_t*)nullptr, n_rec, scan_ws,
                              (int64_t*)nullptr, st)))
         return rc;
-    if (n_rec > 0 && (c.cigar || c.seq || c.qual)) {
+    if (n_rec > 0) {
         unsigned g = (unsigned)((n_rec + FIELDS_NT - 1) / FIELDS_NT);
-        hipLaunchKernelGGL(rec_copy_var, dim3(g), dim3(FIELDS_THREADS), 0, st, data, offsets, n_rec, c);
-        if ((rc = check_launch("rec_copy_var"))) return rc;
+        hipLaunchKernelGGL(rec_columns, dim3(g), dim3(FIELDS_THREADS), 0, st, data, offsets, n_rec, c);
+        if ((rc = check_launch("rec_columns"))) return rc;
     }
     return CBM_OK;
 }

@@ -400,8 +400,8 @@ __device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_
     if (c.tlen) c.tlen[i] = (int32_t)ld_u32(p, o + 32);
     if (c.l_seq) c.l_seq[i] = (int32_t)lseq;
     uint32_t lrn = p[o + 12], ncig = ld_u16(p, o + 16);
-    ncig_out[i] = (int32_t)ncig;
-    lseq_out[i] = (int32_t)lseq;
+    if (ncig_out) ncig_out[i] = (int32_t)ncig;
+    if (lseq_out) lseq_out[i] = (int32_t)lseq;
     int64_t cg = o + 36 + lrn, rlen = 0;
     uint32_t cl = 0, cr = 0;
     for (uint32_t k = 0; k < ncig; k++) {
@@ -513,36 +513,19 @@ __device__ __forceinline__ void stage_records(uint8_t* st, const uint8_t* b, int
     }
 }
 
-__global__ void __launch_bounds__(FIELDS_THREADS) rec_fields(const uint8_t* b, const int64_t* off, int64_t n,
-                                                    cbm_columns c, int32_t* ncig, int32_t* lseq) {
-    __shared__ __align__(16) uint8_t st[SC_STAGE + 16];
-    __shared__ int64_t range[2];
-    const int64_t i0 = (int64_t)blockIdx.x * FIELDS_NT;
-    const int64_t i1 = n < i0 + FIELDS_NT ? n : i0 + FIELDS_NT;
-    const int64_t i = i0 + threadIdx.x;
-    if (threadIdx.x == 0) {
-        const int64_t lo = off[i0], last = off[i1 - 1];
-        const int64_t hi = last + 4 + (int64_t)ld_u32(b, last);
-        // start at a 16-byte aligned address (never below the allocation: device allocations are
-        // 256-byte aligned, and the data pointer lies inside one)
-        const int64_t a0 = lo - (int64_t)(((uintptr_t)(b + lo)) & 15);
-        range[0] = a0;
-        range[1] = (last >= lo && hi - a0 <= SC_STAGE) ? hi : a0;
-    }
-    __syncthreads();
-    const int64_t a0 = range[0], hi = range[1];
-    stage_records(st, b, a0, hi - a0);
-    __syncthreads();
-    if (threadIdx.x >= FIELDS_NT || i >= n) return;
+// n_cigar_op and l_seq of each record, for the CSR offsets (a thread a record)
+__global__ void __launch_bounds__(256) rec_sizes(const uint8_t* b, const int64_t* off, int64_t n, int32_t* ncig,
+                                                 int32_t* lseq) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
     const int64_t o = off[i];
-    bool staged = o >= a0 && o + 36 <= hi;
-    if (staged) staged = o + 4 + (int64_t)ld_u32(st, o - a0) <= hi;
-    if (staged) parse_record(st, o - a0, a0, i, c, ncig, lseq);
-    else parse_record(b, o, 0, i, c, ncig, lseq);
+    ncig[i] = (int32_t)ld_u16(b, o + 16);
+    lseq[i] = (int32_t)ld_u32(b, o + 20);
 }
 
-// CSR columns. A workgroup stages its FIELDS_NT consecutive records in LDS (as rec_fields),
-// then writes each column's output range, which is contiguous across its records: threads
+// The columns. A workgroup stages its FIELDS_NT consecutive records in LDS; each of its first
+// FIELDS_NT threads decodes the fixed fields and tags of one record from there, then the CSR
+// columns are written: each column's output range is contiguous across the records, so threads
 // take consecutive pieces of it, find the record of a piece's first element by a binary
 // search over the records' CSR offsets (in LDS) and read the elements from the stage. Writes
 // are coalesced whatever the records' lengths. Records outside the stage are read in place.
@@ -614,8 +597,8 @@ __device__ __forceinline__ void csr_fill_bytes(uint8_t* out, const int64_t* ooff
     }
 }
 
-__global__ void __launch_bounds__(FIELDS_THREADS) rec_copy_var(const uint8_t* b, const int64_t* off, int64_t n,
-                                                          cbm_columns c) {
+__global__ void __launch_bounds__(FIELDS_THREADS) rec_columns(const uint8_t* b, const int64_t* off, int64_t n,
+                                                         cbm_columns c) {
     __shared__ __align__(16) uint8_t st[SC_STAGE + 16];
     __shared__ int64_t sh_off[FIELDS_NT + 1], sh_src[FIELDS_NT];
     __shared__ int64_t range[2];
@@ -633,6 +616,15 @@ __global__ void __launch_bounds__(FIELDS_THREADS) rec_copy_var(const uint8_t* b,
     const int64_t a0 = range[0], hi = range[1];
     stage_records(st, b, a0, hi - a0);
     __syncthreads();
+    // fixed fields and tags (decode_fixed and find_tag)
+    if (threadIdx.x < m) {
+        const int64_t i = i0 + threadIdx.x, o = off[i];
+        bool staged = o >= a0 && o + 36 <= hi;
+        if (staged) staged = o + 4 + (int64_t)ld_u32(st, o - a0) <= hi;
+        if (staged) parse_record(st, o - a0, a0, i, c, nullptr, nullptr);
+        else parse_record(b, o, 0, i, c, nullptr, nullptr);
+    }
+    if (!(c.cigar && c.cigar_off) && !(c.seq && c.seq_off) && !(c.qual && c.qual_off)) return;
     // where each record's CIGAR, bases and qualities start (data offsets)
     int64_t cig = 0, sq = 0, ql = 0;
     if (threadIdx.x < m) {
@@ -673,7 +665,7 @@ __global__ void __launch_bounds__(FIELDS_THREADS) rec_copy_var(const uint8_t* b,
     }
 }
 
-// per-record sizes for the CSR offsets, from the arrays rec_fields writes
+// per-record sizes for the CSR offsets, from the arrays rec_sizes writes
 struct RecNcig {
     const int32_t* ncig;
     __device__ int64_t operator()(int64_t i) const { return ncig[i]; }
