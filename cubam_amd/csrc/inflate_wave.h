@@ -91,7 +91,7 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 #endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
        STAT_DSTEPS, STAT_LINKIT, STAT_H_STAGE, STAT_H_PARSE, STAT_H_CL, STAT_H_CLDEC, STAT_BLOCKEND, STAT_H_STAGE2,
-       STAT_CLWIN, STAT_DYN, STAT_D_MARK, STAT_D_STEP0, STAT_MBIG, STAT_MSUM, STAT_RSUM, STAT_D_IDLE, STAT_D_LANEIT, STAT_D_WAVEM, STAT_N };
+       STAT_CLWIN, STAT_DYN, STAT_D_MARK, STAT_D_LOOP, STAT_MBIG, STAT_MSUM, STAT_CLLINK, STAT_D_IDLE, STAT_D_LANEIT, STAT_D_WAVEM, STAT_N };
 // the inflate workspace: the block counter, 7 spare words, the statistics, then the match lists
 constexpr int INFLATE_WS_HDR = 512;
 static_assert(8 * (8 + STAT_N) <= INFLATE_WS_HDR, "statistics overrun the workspace header");
@@ -733,7 +733,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                 bad = nb;
                                 entry = want;
                             }
-                            COUNT(STAT_RSUM);  // code-length link iterations
+                            COUNT(STAT_CLLINK);
                             if (!__ballot(changed)) break;
                         }
                         // lanes up to the first that stopped on a bad symbol hold the true walk
@@ -1146,7 +1146,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             COUNT(STAT_DSTEPS);
                         }
                         __syncthreads();
-                        PHASE(STAT_D_STEP0);
+                        PHASE(STAT_D_LOOP);
                     } else {
                         if (tid == 0) {
                             s.sv[SV_F0] = mtotal ? (int)(mm[0] & 0x1ffff) : 0x7fffffff;
