@@ -91,9 +91,16 @@ __device__ inline bool is_record_at(const A& a, int64_t n, int64_t o, int n_ref)
     RecHead h;
     if (!record_head(a, n, o, n_ref, h)) return false;
     int64_t nm = o + 36;
-    for (uint32_t i = 0; i + 1 < h.lrn; i++) {
-        uint32_t c = a.u8(nm + i);
-        if (c < 33 || c > 126 || c == '@') return false;
+    // the name, four characters a read (a read may pass the record's end by up to 3 bytes, which
+    // CBM_PAD keeps readable)
+    const uint32_t nn = h.lrn - 1;
+    for (uint32_t i = 0; i < nn; i += 4) {
+        const uint32_t w = a.u32(nm + i);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t c = (w >> (8 * k)) & 255;
+            if (i + k < nn && (c < 33 || c > 126 || c == '@')) return false;
+        }
     }
     int64_t cg = nm + h.lrn, rlen = 0;
     for (uint32_t i = 0; i < h.ncig; i++) {

@@ -261,15 +261,28 @@ constexpr int RW_STAGE = (int)REC_SEG + RW_OVER + 16;
 
 struct StagedBytes {
     const uint8_t* g;  // the data
-    const uint8_t* l;  // LDS copy of g[base, base + lim)
+    const uint8_t* l;  // LDS copy of g[base, base + lim), 4-byte aligned, readable 4 bytes past lim
     int64_t base, lim;
     __device__ __forceinline__ uint32_t u8(int64_t o) const {
         const int64_t r = o - base;
         return (uint64_t)r < (uint64_t)lim ? l[r] : g[o];
     }
-    __device__ __forceinline__ uint32_t u16(int64_t o) const { return u8(o) | (u8(o + 1) << 8); }
+    // wider reads inside the stage: the two words that hold them, realigned
     __device__ __forceinline__ uint32_t u32(int64_t o) const {
+        const int64_t r = o - base;
+        if ((uint64_t)r + 4 <= (uint64_t)lim) {
+            const uint32_t* w = (const uint32_t*)l + (r >> 2);
+            return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(r & 3));
+        }
         return u8(o) | (u8(o + 1) << 8) | (u8(o + 2) << 16) | (u8(o + 3) << 24);
+    }
+    __device__ __forceinline__ uint32_t u16(int64_t o) const {
+        const int64_t r = o - base;
+        if ((uint64_t)r + 2 <= (uint64_t)lim) {
+            const uint32_t* w = (const uint32_t*)l + (r >> 2);
+            return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(r & 3)) & 0xffffu;
+        }
+        return u8(o) | (u8(o + 1) << 8);
     }
 };
 
@@ -320,7 +333,7 @@ __device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n
 template <bool SPANS>
 __global__ void __launch_bounds__(RW_WAVES * CBM_WAVE)
     rec_walk_wave(const uint8_t* b, int64_t start, int64_t n_all, int n_ref_all, SpanTab t, RecWs w) {
-    __shared__ __align__(16) uint8_t stg[RW_WAVES][RW_STAGE];
+    __shared__ __align__(16) uint8_t stg[RW_WAVES][RW_STAGE + 16];  // 16 spare: word reads past the end
     const int wv = threadIdx.x / CBM_WAVE, lane = threadIdx.x % CBM_WAVE;
     const int64_t k = (int64_t)blockIdx.x * RW_WAVES + wv;
     const bool have = k < w.nseg;  // wave-uniform
