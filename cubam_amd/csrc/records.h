@@ -400,25 +400,62 @@ constexpr int SC_NT = 256;
 // decode_fixed (_kernels.py:292-319) and find_tag (_kernels.py:334-388) of the record at p + o;
 // gbase + (a position in p) is that position in the data (for the tag value offsets). Also
 // writes n_cigar_op and l_seq of the record to ncig[i] and lseq[i] for the CSR offsets.
+// 2- and 4-byte reads at any offset: from LDS as two aligned words realigned (LDS true: p is
+// a 4-byte aligned stage readable 4 bytes past the record), else byte by byte
+template <bool LDS>
+__device__ __forceinline__ uint32_t rd32(const uint8_t* p, int64_t o) {
+    if constexpr (LDS) {
+        const uint32_t* w = (const uint32_t*)p + (o >> 2);
+        return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(o & 3));
+    } else {
+        return ld_u32(p, o);
+    }
+}
+template <bool LDS>
+__device__ __forceinline__ uint32_t rd16(const uint8_t* p, int64_t o) {
+    if constexpr (LDS) return rd32<true>(p, o) & 0xffffu;
+    else return ld_u16(p, o);
+}
+// the first zero byte at or after v, or end (a Z or H value's NUL)
+template <bool LDS>
+__device__ __forceinline__ int64_t zero_after(const uint8_t* p, int64_t v, int64_t end) {
+    if constexpr (LDS) {
+        for (int64_t x = v; x < end; x += 4) {
+            const uint32_t w = rd32<true>(p, x);
+            const uint32_t z = (w - 0x01010101u) & ~w & 0x80808080u;  // high bit of each zero byte
+            if (z) {
+                const int64_t at = x + (__ffs(z) - 1) / 8;
+                return at < end ? at : end;
+            }
+        }
+        return end;
+    } else {
+        int64_t x = v;
+        while (x < end && p[x] != 0) x++;
+        return x;
+    }
+}
+
+template <bool LDS>
 __device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_t gbase, int64_t i,
                                              const cbm_columns& c, int32_t* ncig_out, int32_t* lseq_out) {
-    int32_t pos = (int32_t)ld_u32(p, o + 8);
-    uint32_t lseq = ld_u32(p, o + 20);
-    if (c.ref) c.ref[i] = (int32_t)ld_u32(p, o + 4);
+    int32_t pos = (int32_t)rd32<LDS>(p, o + 8);
+    uint32_t lseq = rd32<LDS>(p, o + 20);
+    if (c.ref) c.ref[i] = (int32_t)rd32<LDS>(p, o + 4);
     if (c.pos) c.pos[i] = pos;
     if (c.mapq) c.mapq[i] = p[o + 13];
-    if (c.flag) c.flag[i] = (uint16_t)ld_u16(p, o + 18);
-    if (c.nref) c.nref[i] = (int32_t)ld_u32(p, o + 24);
-    if (c.npos) c.npos[i] = (int32_t)ld_u32(p, o + 28);
-    if (c.tlen) c.tlen[i] = (int32_t)ld_u32(p, o + 32);
+    if (c.flag) c.flag[i] = (uint16_t)rd16<LDS>(p, o + 18);
+    if (c.nref) c.nref[i] = (int32_t)rd32<LDS>(p, o + 24);
+    if (c.npos) c.npos[i] = (int32_t)rd32<LDS>(p, o + 28);
+    if (c.tlen) c.tlen[i] = (int32_t)rd32<LDS>(p, o + 32);
     if (c.l_seq) c.l_seq[i] = (int32_t)lseq;
-    uint32_t lrn = p[o + 12], ncig = ld_u16(p, o + 16);
+    uint32_t lrn = p[o + 12], ncig = rd16<LDS>(p, o + 16);
     if (ncig_out) ncig_out[i] = (int32_t)ncig;
     if (lseq_out) lseq_out[i] = (int32_t)lseq;
     int64_t cg = o + 36 + lrn, rlen = 0;
     uint32_t cl = 0, cr = 0;
     for (uint32_t k = 0; k < ncig; k++) {
-        uint32_t v = ld_u32(p, cg + 4LL * k), op = v & 15, len = v >> 4;
+        uint32_t v = rd32<LDS>(p, cg + 4LL * k), op = v & 15, len = v >> 4;
         if (ref_consuming(op)) rlen += len;
         if (op == 4) {
             if (k == 0) cl = len;
@@ -431,7 +468,7 @@ __device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_
     int nt = c.n_tags;
     if (nt <= 0) return;
     // aux fields, once for every tag asked for
-    int64_t rec_end = o + 4 + (int64_t)ld_u32(p, o);
+    int64_t rec_end = o + 4 + (int64_t)rd32<LDS>(p, o);
     int64_t a = cg + 4LL * ncig + (lseq + 1) / 2 + lseq;
     uint32_t found = 0;
     bool bad = false;
@@ -447,14 +484,12 @@ __device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_
             case 's': case 'S': nx = v + 2; break;
             case 'i': case 'I': case 'f': nx = v + 4; break;
             case 'Z': case 'H':
-                nx = v;
-                while (nx < rec_end && p[nx] != 0) nx++;
-                nx++;
+                nx = zero_after<LDS>(p, v, rec_end) + 1;
                 break;
             case 'B': {
                 if (v + 5 > rec_end) { nx = -1; break; }
                 uint32_t sub = p[v];
-                int64_t cnt = (int64_t)ld_u32(p, v + 1);
+                int64_t cnt = (int64_t)rd32<LDS>(p, v + 1);
                 int sz = (sub == 'c' || sub == 'C') ? 1 : (sub == 's' || sub == 'S') ? 2 : 4;
                 nx = v + 5 + cnt * sz;
                 break;
@@ -473,10 +508,10 @@ __device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_
                 switch (ty) {
                     case 'c': iv[t] = (int8_t)p[v]; break;
                     case 'C': case 'A': iv[t] = p[v]; break;
-                    case 's': iv[t] = (int16_t)ld_u16(p, v); break;
-                    case 'S': iv[t] = ld_u16(p, v); break;
-                    case 'i': iv[t] = (int32_t)ld_u32(p, v); break;
-                    case 'I': iv[t] = ld_u32(p, v); break;
+                    case 's': iv[t] = (int16_t)rd16<LDS>(p, v); break;
+                    case 'S': iv[t] = rd16<LDS>(p, v); break;
+                    case 'i': iv[t] = (int32_t)rd32<LDS>(p, v); break;
+                    case 'I': iv[t] = rd32<LDS>(p, v); break;
                     case 'Z': case 'H': vl[t] = (int32_t)(nx - 1 - v); break;
                     case 'B': vl[t] = (int32_t)(nx - v); break;
                     default: break;
@@ -634,8 +669,8 @@ __global__ void __launch_bounds__(FIELDS_THREADS) rec_columns(const uint8_t* b, 
         const int64_t i = i0 + threadIdx.x, o = off[i];
         bool staged = o >= a0 && o + 36 <= hi;
         if (staged) staged = o + 4 + (int64_t)ld_u32(st, o - a0) <= hi;
-        if (staged) parse_record(st, o - a0, a0, i, c, nullptr, nullptr);
-        else parse_record(b, o, 0, i, c, nullptr, nullptr);
+        if (staged) parse_record<true>(st, o - a0, a0, i, c, nullptr, nullptr);
+        else parse_record<false>(b, o, 0, i, c, nullptr, nullptr);
     }
     if (!(c.cigar && c.cigar_off) && !(c.seq && c.seq_off) && !(c.qual && c.qual_off)) return;
     // where each record's CIGAR, bases and qualities start (data offsets)
