@@ -140,8 +140,9 @@ class BlockIndex:
         return self.span(lo, min(max(hi, lo + 1), rb.size))
 
     def region_of(self, sp: Span) -> tuple[int, int, int]:
-        """(ref, beg, end) of the record blocks a span starts and ends at (a region that
-        :meth:`region` maps back to a span holding it)."""
+        """(ref, beg, end) of the record blocks a span starts and ends at: a region that
+        :meth:`region` maps back to a span holding it when the span's records lie on one
+        reference (a span that runs into the next reference gets the rest of its first one)."""
         rb = self.record_blocks()
         i0 = int(np.searchsorted(rb, sp.k0))
         i1 = int(np.searchsorted(self.first_rec[rb], sp.end))
