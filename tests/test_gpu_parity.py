@@ -334,15 +334,17 @@ def test_crc32_spans_matches_zlib(cbm):
     assert got == [zlib.crc32(raw[o : o + s].tobytes()) for o, s in zip(off, size)]
 
 
-def test_corrupt_dynamic_headers_match_zlib(cbm, oracle):
-    """Blocks whose dynamic Huffman header (the code-length code and the code lengths) has a few
-    bits flipped, with the trailer of the intact payload, among intact blocks. Against zlib on
+@pytest.mark.parametrize("where", ["header", "anywhere"])
+def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where):
+    """Blocks with a few bits flipped in their dynamic Huffman header (the code-length code and
+    the code lengths: "header") or anywhere in the compressed data, with the trailer of the
+    intact payload, among intact blocks. Against zlib on
     each block: the same blocks inflate; a block that zlib inflates to other bytes (a CRC32
     mismatch) gives the same bytes here and the same error; every block zlib rejects is
     rejected here too (as a Deflate or a size error, which zlib tells apart by where it stopped)."""
     from cubam_amd import _lib
 
-    rng = random.Random(17)
+    rng = random.Random(17 if where == "header" else 18)
     blob, off, size, isz, payloads = bytearray(), [], [], [], []
     for i in range(1200):
         n = rng.randint(200, 65280)
@@ -357,7 +359,7 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle):
         comp = bytearray(c.compress(p) + c.flush())
         if rng.random() < 0.75:
             for _ in range(rng.randint(1, 3)):
-                j = rng.randrange(min(60, len(comp)))
+                j = rng.randrange(min(60, len(comp)) if where == "header" else len(comp))
                 comp[j] ^= 1 << rng.randrange(8)
         head = b"\x1f\x8b\x08\x04\x00\x00\x00\x00\x00\xff\x06\x00BC\x02\x00" + (18 + len(comp) + 7).to_bytes(2, "little")
         block = head + bytes(comp) + (zlib.crc32(p) & 0xFFFFFFFF).to_bytes(4, "little") + len(p).to_bytes(4, "little")
@@ -368,7 +370,7 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle):
         blob += block
     blob += bf.EOF_BLOCK
     arr = np.frombuffer(bytes(blob), np.uint8)
-    _say("corrupt dynamic headers:", len(off), "blocks")
+    _say("corrupt blocks:", where, len(off), "blocks")
     blocks, inf = _gpu_blocks(arr)
     assert blocks.n_blocks == len(off) + 1  # and the EOF block
     st = inf.status.cpu().numpy()[: len(off)] & 255
@@ -378,7 +380,7 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle):
     ok, crc_err = _lib.BLOCK_OK, _lib.BLOCK_CRC
     assert ((st == ok) == (ost == ok)).all(), np.flatnonzero((st == ok) != (ost == ok))[:10]
     assert ((st == crc_err) == (ost == crc_err)).all(), np.flatnonzero((st == crc_err) != (ost == crc_err))[:10]
-    assert (ost == ok).sum() > 200 and (ost == crc_err).sum() > 50 and ((ost != ok) & (ost != crc_err)).sum() > 500
+    assert (ost == ok).sum() > 200 and (ost == crc_err).sum() > 50 and ((ost != ok) & (ost != crc_err)).sum() > 100
     o = 0
     for k in range(len(off)):
         if ost[k] in (ok, crc_err):  # a stream zlib decodes to isize bytes: the same bytes
