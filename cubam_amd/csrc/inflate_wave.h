@@ -142,6 +142,32 @@ __device__ __forceinline__ uint32_t peek32(const S& s, int p, int wbase) {
     return __builtin_amdgcn_alignbit(s.in[w + 1], s.in[w], p & 31);
 }
 
+// Wavefront shifts and scans on the data-parallel primitives (DPP: register moves between
+// lanes, no LDS): lane i gets x of lane i - 1 (lane 0: fill)
+__device__ __forceinline__ int wave_up1(int x, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+}
+// sum (max) of x over lanes 0..i: steps of 1, 2, 4, 8 lanes within rows of 16 (a lane with no
+// source adds 0), then row 15 into rows 1 and 3, then lane 31 into rows 2 and 3
+__device__ __forceinline__ int wave_scan_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111 /* row_shr:1 */, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112 /* row_shr:2 */, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114 /* row_shr:4 */, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118 /* row_shr:8 */, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143 /* row_bcast:31 */, 0xc, 0xf, true);
+    return v;
+}
+__device__ __forceinline__ int wave_scan_max(int v) {  // v >= 0
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, true));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, true));
+    return v;
+}
+
 // one code-length symbol at bit p, looked up in the code-length table (7-bit, no subtables) held
 // in the distance table's space: its bits (code and extra bits; 0 for an invalid code), the
 // symbol (0..18) and how many lengths it gives
@@ -710,8 +736,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         const int a_exit = p, a_bad = bad;
                         int entry = a;
                         while (true) {
-                            int want = __shfl_up(p, 1, CBM_WAVE), wbad = __shfl_up(bad, 1, CBM_WAVE);
-                            if (lane == 0) want = wbase, wbad = 0;
+                            const int want = wave_up1(p, wbase), wbad = wave_up1(bad, 0);
                             bool changed = false;
                             if (!wbad && want != entry) {
                                 int q = want, nb = 0;
@@ -751,14 +776,9 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             }
                         }
                         // offsets (a sum over the lanes before) and the length a leading 16 repeats
-                        int inc = cnt, key = last >= 0 ? ((lane + 1) << 5) | last : 0;
-#pragma unroll
-                        for (int d = 1; d < CBM_WAVE; d <<= 1) {
-                            const int o = __shfl_up(inc, d, CBM_WAVE), k = __shfl_up(key, d, CBM_WAVE);
-                            if (lane >= d) inc += o, key = k > key ? k : key;
-                        }
-                        int kex = __shfl_up(key, 1, CBM_WAVE);
-                        if (lane == 0) kex = 0;
+                        const int inc = wave_scan_sum(cnt);
+                        const int key = wave_scan_max(last >= 0 ? ((lane + 1) << 5) | last : 0);
+                        const int kex = wave_up1(key, 0);
                         int n = n0 + inc - cnt, prev = kex ? (kex & 31) : prev0, myerr = 0, myend = -1;
                         if (alive && n < total) {
                             int q = entry;
@@ -798,14 +818,14 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         }
                         const uint64_t em = __ballot(myend >= 0);
                         if (em) {
-                            endp = __shfl(myend, __ffsll((unsigned long long)em) - 1, CBM_WAVE);
+                            endp = __builtin_amdgcn_readlane(myend, __ffsll((unsigned long long)em) - 1);
                             break;
                         }
                         // every lane's walk is true and the lengths go on past this pass
-                        n0 = __shfl(n0 + inc, CBM_WAVE - 1, CBM_WAVE);
-                        const int k63 = __shfl(key, CBM_WAVE - 1, CBM_WAVE);
+                        n0 = __builtin_amdgcn_readlane(n0 + inc, CBM_WAVE - 1);
+                        const int k63 = __builtin_amdgcn_readlane(key, CBM_WAVE - 1);
                         prev0 = k63 ? (k63 & 31) : prev0;
-                        wbase = __shfl(p, CBM_WAVE - 1, CBM_WAVE);
+                        wbase = __builtin_amdgcn_readlane(p, CBM_WAVE - 1);
                     }
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_s_setprio(0);
