@@ -45,12 +45,15 @@ constexpr int md_k() { return NT >= 512 ? 4 : 8; }
 
 template <int NT, int SEG>
 struct WaveSmem {
-    static constexpr int IN_WORDS = NT * SEG / 32 + 8;  // staged compressed bits of a round
+    // staged compressed bits of a round; at least 1 << LBITS words, the table build's scratch
+    static constexpr int IN_WORDS = NT * SEG / 32 + 8 > (1 << LBITS) + 8 ? NT * SEG / 32 + 8 : (1 << LBITS) + 8;
     static_assert(IN_WORDS >= 256, "the staging area holds the CRC32 byte table at block end");
-    uint8_t out[65536 + 16];
-    uint32_t in[IN_WORDS];
-    uint16_t lit[LTAB];
+    static_assert(IN_WORDS >= (1 << LBITS), "the staging area is the table build's scratch");
+    // the tables and the staging area first: their offsets fit the 16-bit offset field of the
+    // LDS instructions, so a lookup needs no address add
+    uint32_t lit[LTAB];
     uint16_t dist[DTAB];
+    uint32_t in[IN_WORDS];
     union {
         struct {  // header and table build
             uint16_t code[320];
@@ -67,6 +70,7 @@ struct WaveSmem {
     int32_t lnext[16];
     int32_t chunkcnt[5][16];
     int32_t sv[SV_N];  // shared scalars (SV_*)
+    alignas(16) uint8_t out[65536 + 16];
 };
 
 
@@ -126,13 +130,10 @@ __device__ __forceinline__ void dist_code(int d, int& base, int& extra) {
 // 64 bits of the staged stream from bit p (p relative to the payload start)
 template <class S>
 __device__ __forceinline__ uint64_t peek64(const S& s, int p, int wbase) {
-    int w = (p >> 5) - wbase;
-    uint32_t lo = s.in[w], mid = s.in[w + 1], hi = s.in[w + 2];
-    int sh = p & 31;
-    uint64_t v = ((uint64_t)mid << 32) | lo;
-    v >>= sh;
-    if (sh) v |= (uint64_t)hi << (64 - sh);
-    return v;
+    const int w = (p >> 5) - wbase;
+    const uint32_t lo = s.in[w], mid = s.in[w + 1], hi = s.in[w + 2];
+    const uint32_t sh = (uint32_t)p & 31;
+    return ((uint64_t)__builtin_amdgcn_alignbit(hi, mid, sh) << 32) | __builtin_amdgcn_alignbit(mid, lo, sh);
 }
 
 // 32 bits of the staged stream from bit p
@@ -199,73 +200,62 @@ struct Sym {
     int dist;
 };
 
+// Decode tables of this variant. Literal/length entries are 32 bits:
+//   leaf:    bit 31 = 0, bits 0-3 = code length (0: no code), bits 4-6 = extra bits of a length,
+//            bits 8-9 = kind (0 literal, 1 length, 2 end of block, 3 symbol 286 or 287),
+//            bits 16-24 = the literal byte or the length's base (RFC 1951 3.2.5)
+//   pointer: bit 31 = 1, bits 0-3 = index bits of the subtable, bits 4-14 = its offset
+// Distance entries are 16 bits:
+//   leaf:    bit 15 = 0, bits 0-3 = code length, bits 4-7 = extra bits, bits 8-12 = symbol
+//   pointer: bit 15 = 1, bits 0-3 = index bits, bits 4-14 = offset
+// A subtable leaf holds the whole code length, root bits included. The code-length code's table
+// keeps bits 0-3 = length and bits 4-12 = symbol.
+constexpr uint32_t KIND_LIT = 0, KIND_LEN = 1, KIND_EOB = 2, KIND_BAD = 3;
+
+__device__ __forceinline__ uint32_t lit_entry(int sym, int len) {
+    if (sym < 256) return ((uint32_t)sym << 16) | (uint32_t)len;
+    if (sym == 256) return (KIND_EOB << 8) | (uint32_t)len;
+    if (sym > 285) return (KIND_BAD << 8) | (uint32_t)len;
+    int base, extra;
+    len_code(sym - 257, base, extra);
+    return ((uint32_t)base << 16) | (KIND_LEN << 8) | ((uint32_t)extra << 4) | (uint32_t)len;
+}
+__device__ __forceinline__ uint32_t dist_entry(int sym, int len) {
+    const int extra = sym < 4 ? 0 : (sym >> 1) - 1;  // symbols 30 and 31 are rejected by the decoder
+    return ((uint32_t)sym << 8) | ((uint32_t)extra << 4) | (uint32_t)len;
+}
+
 // decode one symbol (a literal/length code and, for a length, its distance) from the next
-// 64 bits of the stream; a symbol uses at most 15 + 5 + 15 + 13 = 48 bits
+// 64 bits of the stream; a symbol uses at most 15 + 5 + 15 + 13 = 48 bits. A code the table
+// does not hold, and distance symbols 30 and 31, give kind 3.
 template <class S>
 __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
+    const uint32_t v0 = (uint32_t)v;
+    uint32_t e = s.lit[v0 & ((1u << LBITS) - 1)];
+    if ((int32_t)e < 0) e = s.lit[((e >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(v0, LBITS, e & 15)];
+    const int cl = (int)(e & 15);
     Sym r;
-    uint32_t e = s.lit[v & ((1u << LBITS) - 1)];
-    int used;
-    if (e & 0x8000) {
-        int m = e & 15, off = (e >> 4) & 0x7ff;
-        uint32_t e2 = s.lit[off + ((uint32_t)(v >> LBITS) & ((1u << m) - 1))];
-        used = LBITS + (e2 & 15);
-        e = (e2 & 15) ? e2 : 0;
-    } else {
-        used = e & 15;
+    r.kind = cl ? (int)((e >> 8) & 3) : (int)KIND_BAD;
+    r.bits = cl ? cl : 1;
+    r.val = (int)(e >> 16);
+    r.dist = 0;
+    if (r.kind == (int)KIND_LEN) {
+        const int ex = (int)((e >> 4) & 7), u = cl + ex;
+        r.val += (int)__builtin_amdgcn_ubfe(v0, cl, ex);
+        const uint32_t x = (uint32_t)(v >> u);
+        uint32_t d = s.dist[x & ((1u << DBITS) - 1)];
+        if (d & 0x8000) d = s.dist[((d >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(x, DBITS, d & 15)];
+        const int dl = (int)(d & 15), de = (int)((d >> 4) & 15), ds = (int)((d >> 8) & 31);
+        if (!dl || ds >= 30) {
+            r.kind = (int)KIND_BAD;
+            r.bits = u + (dl ? dl : 1);
+        } else {
+            int db, de2;
+            dist_code(ds, db, de2);
+            r.bits = u + dl + de;
+            r.dist = db + (int)__builtin_amdgcn_ubfe(x, dl, de);
+        }
     }
-    if (!(e & 15)) {
-        r.kind = 3;
-        r.bits = 1;
-        return r;
-    }
-    int sym = (e >> 4) & 0x1ff;
-    if (sym < 256) {
-        r.kind = 0;
-        r.bits = used;
-        r.val = sym;
-        return r;
-    }
-    if (sym == 256) {
-        r.kind = 2;
-        r.bits = used;
-        return r;
-    }
-    if (sym > 285) {
-        r.kind = 3;
-        r.bits = used;
-        return r;
-    }
-    int lb, le;
-    len_code(sym - 257, lb, le);
-    v >>= used;
-    int len = lb + (int)(v & ((1u << le) - 1));
-    v >>= le;
-    used += le;
-    uint32_t d = s.dist[v & ((1u << DBITS) - 1)];
-    int dused;
-    if (d & 0x8000) {
-        int m = d & 15, off = (d >> 4) & 0x7ff;
-        uint32_t d2 = s.dist[off + ((uint32_t)(v >> DBITS) & ((1u << m) - 1))];
-        dused = DBITS + (d2 & 15);
-        d = (d2 & 15) ? d2 : 0;
-    } else {
-        dused = d & 15;
-    }
-    int ds = (d >> 4) & 0x1ff;
-    if (!(d & 15) || ds >= 30) {
-        r.kind = 3;
-        r.bits = used + (dused ? dused : 1);
-        return r;
-    }
-    v >>= dused;
-    used += dused;
-    int db, de;
-    dist_code(ds, db, de);
-    r.kind = 1;
-    r.val = len;
-    r.dist = db + (int)(v & ((1u << de) - 1));
-    r.bits = used + de;
     return r;
 }
 
@@ -298,9 +288,10 @@ __device__ __forceinline__ Sym win_next(Win& w, const S& s, int wb) {
 // ------------------------------------------------------------------ cooperative table build
 // Canonical Huffman decode table of lens[0:nsym) into tab (root bits, cap entries), with the
 // checks of build_table; scratch32 holds 1 << root words. Returns (in all threads) success.
-template <int NT, class S>
-__device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, uint16_t* tab, int cap, int type,
+template <int NT, class S, class T>
+__device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, T* tab, int cap, int type,
                            uint32_t* scratch32) {
+    constexpr uint32_t PTR = sizeof(T) == 4 ? 0x80000000u : 0x8000u;
     const int tid = threadIdx.x, lane = tid % CBM_WAVE, wave = tid / CBM_WAVE;
     const int rsize = 1 << root;
     if (tid < 16) s.lcount[tid] = 0;
@@ -373,7 +364,7 @@ __device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, uint16
         for (int i = lo; i < hi; i++) {
             int m = scratch32[i];
             if (m) {
-                tab[i] = (uint16_t)(0x8000 | (off << 4) | m);
+                tab[i] = (T)(PTR | ((uint32_t)off << 4) | (uint32_t)m);
                 off += 1 << m;
             }
         }
@@ -383,13 +374,12 @@ __device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, uint16
         int L = lens[i];
         if (!L) continue;
         uint32_t rev = bitrev(s.code[i], L);
+        const T e = (T)(type == TAB_LENS ? lit_entry(i, L) : type == TAB_DISTS ? dist_entry(i, L) : (uint32_t)((i << 4) | L));
         if (L <= root) {
-            uint16_t e = (uint16_t)((i << 4) | L);
             for (uint32_t j = rev; j < (uint32_t)rsize; j += 1u << L) tab[j] = e;
         } else {
-            uint16_t p = tab[rev & (rsize - 1)];
-            int m = p & 15, off = (p >> 4) & 0x7ff, sl = L - root;
-            uint16_t e = (uint16_t)((i << 4) | sl);
+            const uint32_t p = tab[rev & (rsize - 1)];
+            const int m = p & 15, off = (p >> 4) & 0x7ff, sl = L - root;
             for (uint32_t j = rev >> root; j < (1u << m); j += 1u << sl) tab[off + j] = e;
         }
     }
