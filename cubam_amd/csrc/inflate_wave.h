@@ -9,8 +9,8 @@
 // Pass A decodes speculatively, from SEG bits before the segment (so that the walk
 // is most likely on the true boundaries when it enters it) to the first symbol
 // that starts at or after the next segment, and keeps checkpoints inside its
-// segment: the first symbol start at or after every SEG / NCP-th bit, with the
-// output bytes and matches counted up to there. The walks are then linked: thread
+// segment: the symbol start before every few symbols, with the output bytes and
+// matches counted up to there. The walks are then linked: thread
 // t's true entry is thread t-1's exit; a thread whose entry changed decodes again
 // from it until it lands on a checkpoint (from there its first walk was right, and
 // its count from there is known) - repeated until nothing changes. A workgroup
@@ -89,6 +89,10 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 // starts before its segment
 #ifndef CBM_INFLATE_FWD_PASSES
 #define CBM_INFLATE_FWD_PASSES 2
+#endif
+// symbols between two checkpoints of a speculative walk
+#ifndef CBM_INFLATE_CP_STEPS
+#define CBM_INFLATE_CP_STEPS 3
 #endif
 #ifndef CBM_INFLATE_OVL_HALVES
 #define CBM_INFLATE_OVL_HALVES 2
@@ -865,10 +869,11 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // walk has most likely fallen onto the true symbol boundaries when it enters the
                 // segment. A walk that stops (end of block or a bad code) before its segment starts
                 // again from the segment start. Inside the segment the walk keeps NCP checkpoints:
-                // the first symbol start at or after a + j * SEG / NCP, with the output bytes and
-                // matches counted from checkpoint 0 up to there, and in all to its exit.
+                // checkpoint 0 is its first symbol start at or after a, and checkpoint j the start
+                // j * CP_STEPS symbols later, each with the output bytes and matches counted from
+                // checkpoint 0 up to there; and the count to its exit.
                 constexpr int OVL = SEG * CBM_INFLATE_OVL_HALVES / 2;
-                constexpr int NCP = SEG / 32 < 8 ? SEG / 32 : 8, CPS = SEG / NCP;
+                constexpr int NCP = SEG / 32 < 8 ? SEG / 32 : 8, CP_STEPS = CBM_INFLATE_CP_STEPS;
                 int cp[NCP];
                 uint32_t cum[NCP];
 #pragma unroll
@@ -878,29 +883,30 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 if (active) {
                     Win w;
                     win_at(w, s, a - OVL > base ? a - OVL : base, wb);
-                    int nj = 0, nxt = a;  // the next checkpoint, and the bit it starts at
-                    while (w.p < aend) {
-                        const int at = w.p;
-                        if (at >= nxt) {  // at most NCP times a walk
-                            do {
-#pragma unroll
-                                for (int j = 0; j < NCP; j++)
-                                    if (j == nj) cp[j] = at, cum[j] = run;
-                                nj++;
-                                nxt += CPS;
-                            } while (at >= nxt && nj < NCP);
-                            if (nj == NCP) nxt = 0x7fffffff;
-                        }
-                        Sym y = win_next(w, s, wb);
-                        const bool halt = y.kind == 2 || y.kind == 3 || w.p > in_bits;
-                        if (halt && at < a) {
+                    // the overlap: symbol lengths only, up to the first symbol start at or after a
+                    while (w.p < a) {
+                        const Sym y = win_next(w, s, wb);
+                        if (y.kind >= 2 || w.p > in_bits) {
                             win_at(w, s, a, wb);
-                            continue;
+                            break;
                         }
-                        if (nj) run += y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
-                        if (y.kind == 2) { stop = STOP_EOB; break; }
-                        if (halt) { stop = STOP_BAD; break; }
                     }
+                    // the segment, with a checkpoint every CP_STEPS symbols (the same step for all
+                    // lanes of a wavefront, so recording one costs no test per symbol)
+                    bool walking = w.p < aend;
+                    auto step = [&]() {
+                        const Sym y = win_next(w, s, wb);
+                        run += y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
+                        if (y.kind == 2) stop = STOP_EOB;
+                        else if (y.kind == 3 || w.p > in_bits) stop = STOP_BAD;
+                        walking = stop == STOP_NONE && w.p < aend;
+                    };
+#pragma unroll
+                    for (int j = 0; j < NCP; j++) {
+                        if (walking) cp[j] = w.p, cum[j] = run;
+                        for (int k = 0; k < CP_STEPS && walking; k++) step();
+                    }
+                    while (walking) step();
                     p = w.p;
                 }
                 s.exit_[tid] = p;
