@@ -250,41 +250,40 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
         uint32_t d = s.dist[x & ((1u << DBITS) - 1)];
         if (d & 0x8000) d = s.dist[((d >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(x, DBITS, d & 15)];
         const int dl = (int)(d & 15), de = (int)((d >> 4) & 15), ds = (int)((d >> 8) & 31);
-        if (!dl || ds >= 30) {
-            r.kind = (int)KIND_BAD;
-            r.bits = u + (dl ? dl : 1);
-        } else {
-            int db, de2;
-            dist_code(ds, db, de2);
-            r.bits = u + dl + de;
-            r.dist = db + (int)__builtin_amdgcn_ubfe(x, dl, de);
-        }
+        const bool bad = !dl || ds >= 30;
+        int db, de2;
+        dist_code(ds, db, de2);
+        r.kind = bad ? (int)KIND_BAD : (int)KIND_LEN;
+        r.bits = bad ? u + (dl ? dl : 1) : u + dl + de;
+        r.dist = db + (int)__builtin_amdgcn_ubfe(x, dl, de);
     }
     return r;
 }
 
-// A window on the staged stream: decodes symbol after symbol, re-reading LDS only when fewer
-// than 48 bits are left (after a literal of at most 15 bits, the next symbol still fits)
+// the stop a symbol causes: STOP_EOB at the end of block, STOP_BAD for a bad code or a symbol
+// that ends past the payload, else STOP_NONE
+__device__ __forceinline__ int sym_stop(const Sym& y, int p, int in_bits) {
+    const int k = y.kind >= 2 ? y.kind - 1 : STOP_NONE;
+    return k == STOP_NONE && p > in_bits ? STOP_BAD : k;
+}
+// a symbol's output as a walk counts it: matches << 20 | bytes
+__device__ __forceinline__ uint32_t sym_count(const Sym& y) {
+    return y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
+}
+
+// A walk's position on the staged stream. Each symbol is decoded from 64 bits read afresh from
+// LDS at the position (three words, two funnel shifts): cheaper than keeping a register window
+// and testing its fill every step.
 struct Win {
-    int p;      // stream position of v's bit 0
-    int avail;  // valid bits in v
-    uint64_t v;
+    int p;
 };
 template <class S>
-__device__ __forceinline__ void win_at(Win& w, const S& s, int p, int wb) {
+__device__ __forceinline__ void win_at(Win& w, const S&, int p, int) {
     w.p = p;
-    w.v = peek64(s, p, wb);
-    w.avail = 64;
 }
 template <class S>
 __device__ __forceinline__ Sym win_next(Win& w, const S& s, int wb) {
-    if (w.avail < 48) {
-        w.v = peek64(s, w.p, wb);
-        w.avail = 64;
-    }
-    Sym y = decode_v(s, w.v);
-    w.v >>= y.bits;
-    w.avail -= y.bits;
+    const Sym y = decode_v(s, peek64(s, w.p, wb));
     w.p += y.bits;
     return y;
 }
@@ -896,9 +895,8 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     bool walking = w.p < aend;
                     auto step = [&]() {
                         const Sym y = win_next(w, s, wb);
-                        run += y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
-                        if (y.kind == 2) stop = STOP_EOB;
-                        else if (y.kind == 3 || w.p > in_bits) stop = STOP_BAD;
+                        run += sym_count(y);
+                        stop = sym_stop(y, w.p, in_bits);
                         walking = stop == STOP_NONE && w.p < aend;
                     };
 #pragma unroll
@@ -942,9 +940,9 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                                 if (w.p == cp[j]) sj = j;
                             if (sj >= 0) break;
                             Sym y = win_next(w, s, wb);
-                            r2 += y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
-                            if (y.kind == 2) { nstop = STOP_EOB; break; }
-                            if (y.kind == 3 || w.p > in_bits) { nstop = STOP_BAD; break; }
+                            r2 += sym_count(y);
+                            nstop = sym_stop(y, w.p, in_bits);
+                            if (nstop != STOP_NONE) break;
                         }
                         int np = w.p;
                         if (sj >= 0) {
@@ -1004,16 +1002,16 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                     Win w;
                     win_at(w, s, tid == 0 ? base : s.exit_[tid - 1], wb);
                     while (w.p < e) {
-                        Sym y = win_next(w, s, wb);
-                        if (y.kind == 0) {
-                            s.out[o++] = (uint8_t)y.val;
-                        } else if (y.kind == 1) {
-                            if (y.dist > o) { bad = true; break; }
-                            mlist[k++] = (uint64_t)o | ((uint64_t)y.dist << 17) | ((uint64_t)y.val << 33);
-                            o += y.val;
-                        } else {
+                        const Sym y = win_next(w, s, wb);
+                        if (y.kind >= 2 || (y.kind == 1 && y.dist > o)) {
+                            bad = y.kind == 1;  // a distance reaching before the block's first byte
                             break;
                         }
+                        if (y.kind == 0) s.out[o] = (uint8_t)y.val;
+                        else
+                            mlist[k++] = (uint64_t)(uint32_t)o | ((uint64_t)(uint32_t)y.dist << 17) |
+                                         ((uint64_t)(uint32_t)y.val << 33);
+                        o += y.kind == 0 ? 1 : y.val;
                     }
                     if (bad) {
                         atomicOr(&s.sv[SV_ERR], CBM_BLOCK_DEFLATE);
