@@ -95,7 +95,7 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 #define CBM_INFLATE_CP_STEPS 3
 #endif
 #ifndef CBM_INFLATE_OVL_HALVES
-#define CBM_INFLATE_OVL_HALVES 2
+#define CBM_INFLATE_OVL_HALVES 3
 #endif
 enum { STAT_HEADER = 0, STAT_PASS_A, STAT_LINK, STAT_PASS_B, STAT_PASS_C, STAT_PASS_D, STAT_STORE, STAT_ROUNDS,
        STAT_DSTEPS, STAT_LINKIT, STAT_H_STAGE, STAT_H_PARSE, STAT_H_CL, STAT_H_CLDEC, STAT_BLOCKEND, STAT_H_STAGE2,
@@ -872,7 +872,11 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // j * CP_STEPS symbols later, each with the output bytes and matches counted from
                 // checkpoint 0 up to there; and the count to its exit.
                 constexpr int OVL = SEG * CBM_INFLATE_OVL_HALVES / 2;
+#ifdef CBM_INFLATE_NCP
+                constexpr int NCP = CBM_INFLATE_NCP, CP_STEPS = CBM_INFLATE_CP_STEPS;
+#else
                 constexpr int NCP = SEG / 32 < 8 ? SEG / 32 : 8, CP_STEPS = CBM_INFLATE_CP_STEPS;
+#endif
                 int cp[NCP];
                 uint32_t cum[NCP];
 #pragma unroll
@@ -1274,9 +1278,14 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 uint32_t x = 0xFFFFFFFFu;
                 int k = s2;
                 if constexpr (CRC_TABS == 4) {
+                    // four bytes a step, read as aligned words realigned by the slice's offset
+                    const uint32_t* ow = (const uint32_t*)s.out;
+                    const uint32_t sh = (uint32_t)s2 & 3;
+                    uint32_t lo = ow[s2 >> 2];
                     for (; k + 4 <= e2; k += 4) {
-                        x ^= (uint32_t)s.out[k] | ((uint32_t)s.out[k + 1] << 8) | ((uint32_t)s.out[k + 2] << 16) |
-                             ((uint32_t)s.out[k + 3] << 24);
+                        const uint32_t hi = ow[(k >> 2) + 1];
+                        x ^= __builtin_amdgcn_alignbyte(hi, lo, sh);
+                        lo = hi;
                         x = crcT[768 + (x & 255)] ^ crcT[512 + ((x >> 8) & 255)] ^ crcT[256 + ((x >> 16) & 255)] ^
                             crcT[x >> 24];
                     }
