@@ -480,7 +480,10 @@ __device__ __forceinline__ bool any_bits(const uint32_t* bm, int a, int b) {
 
 // pass D, per wavefront: more lanes with ready matches than this and each lane copies its own;
 // idle polls (with a short sleep each) before a round is given up as bad
-constexpr int D_LANE_COPY = 8;
+#ifndef CBM_D_LANE_COPY
+#define CBM_D_LANE_COPY 8
+#endif
+constexpr int D_LANE_COPY = CBM_D_LANE_COPY;
 constexpr int D_IDLE_MAX = 1 << 16;
 
 // Are the bytes match m (dest | dist << 17 | len << 33) reads final: is none of them (at or after
