@@ -88,7 +88,7 @@ enum { STOP_NONE = 0, STOP_EOB = 1, STOP_BAD = 2 };
 // passes of match forwarding in pass D, and the bits (in half segments) a speculative walk
 // starts before its segment
 #ifndef CBM_INFLATE_FWD_PASSES
-#define CBM_INFLATE_FWD_PASSES 2
+#define CBM_INFLATE_FWD_PASSES 3
 #endif
 // symbols between two checkpoints of a speculative walk
 #ifndef CBM_INFLATE_CP_STEPS
@@ -925,54 +925,64 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 const uint32_t a_run = run;
                 __syncthreads();
                 PHASE(STAT_PASS_A);
-                // link the walks
-                if (tid == 0) s.sv[SV_F0] = s.sv[SV_F1] = s.sv[SV_F2] = 0;
+                // link the walks. Each wavefront runs its own links to a fixed point, lane t taking
+                // lane t-1's exit by a DPP shift, with no workgroup barrier; only exits that cross a
+                // wavefront boundary go through LDS, and a pass repeats while one of them moved.
+                const int lane = tid % CBM_WAVE;
+                int pub = active ? stop : STOP_BAD;  // the stop a successor sees
+                if (tid == 0) s.sv[SV_F0] = s.sv[SV_F1] = 0;
                 __syncthreads();
                 for (int it = 0;; it++) {
-                    // flag it % 3 gathers this iteration's changes; it + 1 was cleared last time
-                    int want = tid == 0 ? base : s.exit_[tid - 1];
-                    int pstop = tid == 0 ? STOP_NONE : s.stop_[tid - 1];
-                    if (tid == 0) s.sv[SV_F0 + (it + 1) % 3] = 0;
-                    __syncthreads();
-                    const bool redo = active && pstop == STOP_NONE && want != entry;
-                    if (redo) {
-                        // walk from the new entry until a checkpoint of the first walk
-                        int nstop = STOP_NONE, sj = -1;
-                        uint32_t r2 = 0;
-                        Win w;
-                        win_at(w, s, want, wb);
-                        while (w.p < aend) {
+                    const int w0 = tid == 0 ? base : s.exit_[tid - 1];
+                    const int ps0 = tid == 0 ? STOP_NONE : s.stop_[tid - 1];
+                    for (;;) {
+                        const int want = wave_up1(p, w0);
+                        const int pstop = wave_up1(pub, ps0);
+                        const bool redo = active && pstop == STOP_NONE && want != entry;
+                        if (!__ballot(redo)) break;
+                        if (redo) {
+                            // walk from the new entry until a checkpoint of the first walk
+                            int nstop = STOP_NONE, sj = -1;
+                            uint32_t r2 = 0;
+                            Win w;
+                            win_at(w, s, want, wb);
+                            while (w.p < aend) {
 #pragma unroll
-                            for (int j = 0; j < NCP; j++)
-                                if (w.p == cp[j]) sj = j;
-                            if (sj >= 0) break;
-                            Sym y = win_next(w, s, wb);
-                            r2 += sym_count(y);
-                            nstop = sym_stop(y, w.p, in_bits);
-                            if (nstop != STOP_NONE) break;
-                        }
-                        int np = w.p;
-                        if (sj >= 0) {
-                            uint32_t c = 0;
+                                for (int j = 0; j < NCP; j++)
+                                    if (w.p == cp[j]) sj = j;
+                                if (sj >= 0) break;
+                                Sym y = win_next(w, s, wb);
+                                r2 += sym_count(y);
+                                nstop = sym_stop(y, w.p, in_bits);
+                                if (nstop != STOP_NONE) break;
+                            }
+                            int np = w.p;
+                            if (sj >= 0) {
+                                uint32_t c = 0;
 #pragma unroll
-                            for (int j = 0; j < NCP; j++)
-                                if (j == sj) c = cum[j];
-                            np = a_exit;
-                            nstop = a_stop;
-                            r2 += a_run - c;
+                                for (int j = 0; j < NCP; j++)
+                                    if (j == sj) c = cum[j];
+                                np = a_exit;
+                                nstop = a_stop;
+                                r2 += a_run - c;
+                            }
+                            entry = want;
+                            run = r2;
+                            p = np;
+                            stop = nstop;
+                            pub = stop;
                         }
-                        entry = want;
-                        run = r2;
-                        if (np != p || nstop != stop) s.sv[SV_F0 + it % 3] = 1;
-                        p = np;
-                        stop = nstop;
-                        s.exit_[tid] = p;
-                        s.stop_[tid] = stop;
+                        COUNT(STAT_LINKIT);
                     }
+                    __syncthreads();  // every read of the exits of this pass is done
+                    s.exit_[tid] = p;
+                    s.stop_[tid] = pub;
+                    if (tid == 0) s.sv[SV_F0 + (it + 1) % 2] = 0;
                     __syncthreads();
-                    const int changed = s.sv[SV_F0 + it % 3];
-                    COUNT(STAT_LINKIT);
-                    if (!changed) break;
+                    if (lane == 0 && tid > 0 && (s.exit_[tid - 1] != w0 || s.stop_[tid - 1] != ps0))
+                        s.sv[SV_F0 + it % 2] = 1;
+                    __syncthreads();
+                    if (!s.sv[SV_F0 + it % 2]) break;
                 }
                 PHASE(STAT_LINK);
                 // the first walk that stops ends the round (and the Huffman block, on EOB)
