@@ -864,8 +864,15 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 const int wb = base >> 5;
                 stage<NT>(s, in, wb, limit);
                 __syncthreads();
-                const int a = base + tid * SEG;         // speculative start
-                const int aend = a + SEG;
+                // segment bits this round: SEG, or fewer in the last round of the final Huffman
+                // block, where the payload's end bounds the bits left (a multiple of 32 bits that
+                // covers them, so every thread has a share)
+                const int bits_left = in_bits - base;
+                const int seg = s.sv[SV_FINAL] && bits_left < NT * SEG
+                                    ? max(32, ((bits_left + NT - 1) / NT + 31) & ~31)
+                                    : SEG;
+                const int a = base + tid * seg;         // speculative start
+                const int aend = a + seg;
                 const bool active = a < in_bits;        // thread 0 is always active
                 // pass A. Threads after the first start OVL bits before their segment, so that the
                 // walk has most likely fallen onto the true symbol boundaries when it enters the
@@ -874,7 +881,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // checkpoint 0 is its first symbol start at or after a, and checkpoint j the start
                 // j * CP_STEPS symbols later, each with the output bytes and matches counted from
                 // checkpoint 0 up to there; and the count to its exit.
-                constexpr int OVL = SEG * CBM_INFLATE_OVL_HALVES / 2;
+                const int OVL = seg * CBM_INFLATE_OVL_HALVES / 2;
 #ifdef CBM_INFLATE_NCP
                 constexpr int NCP = CBM_INFLATE_NCP, CP_STEPS = CBM_INFLATE_CP_STEPS;
 #else
