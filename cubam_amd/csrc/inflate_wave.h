@@ -179,7 +179,11 @@ __device__ __forceinline__ int wave_scan_max(int v) {  // v >= 0
 struct ClSym {
     int bits, sym, rep;
 };
-constexpr int CL_SEG = 32;  // bits a lane takes: more than the 14 of the longest symbol
+#ifndef CBM_CL_SEG
+#define CBM_CL_SEG 24
+#endif
+constexpr int CL_SEG = CBM_CL_SEG;  // bits a lane takes: more than the 14 of the longest symbol
+static_assert(CL_SEG >= 15 && CL_SEG <= 32, "a lane's walk records its starts in 32 bits and covers a symbol");
 
 // the symbol at bit r of a lane's window (win holds 64 staged bits from the lane's first bit; a
 // symbol starts at r < CL_SEG and takes at most 14 bits)
