@@ -520,19 +520,32 @@ __device__ __forceinline__ void wave_copy_match(uint8_t* out, uint64_t m, int la
 
 // ------------------------------------------------------------------ staging
 // Stage payload words [w0, w0 + IN_WORDS) (word = 4 payload bytes; payload byte 0 = in[0])
-// into LDS; bytes at or past `limit` (payload bytes + trailer) read as zero.
+// into LDS; bytes at or past `limit` (payload bytes + trailer) read as zero. A thread issues all
+// its loads before it writes any word, so a stage costs one global-memory latency, not one per
+// word; the aligned source pointer is derived from `in` so the loads stay global loads.
 template <int NT, class S>
 __device__ void stage(S& s, const uint8_t* in, int w0, int limit) {
-    uintptr_t a = (uintptr_t)in;
-    const uint32_t* al = (const uint32_t*)(a & ~(uintptr_t)3);
-    int sh = (int)(a & 3);
-    for (int i = threadIdx.x; i < S::IN_WORDS; i += NT) {
-        int byte0 = 4 * (w0 + i);
+    const int sh = (int)((uintptr_t)in & 3);
+    const uint32_t* al = (const uint32_t*)(in - sh);
+    constexpr int PER = (S::IN_WORDS + NT - 1) / NT;
+    uint32_t x[PER], y[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int i = (int)threadIdx.x + u * NT, byte0 = 4 * (w0 + i);
+        x[u] = 0, y[u] = 0;
+        if (i < S::IN_WORDS && byte0 + 4 <= limit) {
+            const int k = (byte0 + sh) >> 2;
+            x[u] = al[k];
+            if (sh) y[u] = al[k + 1];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int i = (int)threadIdx.x + u * NT, byte0 = 4 * (w0 + i);
+        if (i >= S::IN_WORDS) break;
         uint32_t w = 0;
         if (byte0 + 4 <= limit) {
-            int k = (byte0 + sh) >> 2;
-            uint32_t x0 = al[k];
-            w = sh ? ((x0 >> (8 * sh)) | (al[k + 1] << (32 - 8 * sh))) : x0;
+            w = sh ? __builtin_amdgcn_alignbit(y[u], x[u], (uint32_t)(8 * sh)) : x[u];
         } else if (byte0 < limit) {
             for (int j = 0; j < 4 && byte0 + j < limit; j++) w |= (uint32_t)in[byte0 + j] << (8 * j);
         }
