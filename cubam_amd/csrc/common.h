@@ -7,6 +7,10 @@
 #include "../../include/cubam_amd.h"
 
 #define CBM_WAVE 64
+// pointers whose address space the compiler keeps: a read through one is an LDS read or a global
+// load, never a flat load (which waits on both counters)
+#define CBM_LDS __attribute__((address_space(3)))
+#define CBM_GLOBAL __attribute__((address_space(1)))
 
 namespace cbm {
 

@@ -260,18 +260,19 @@ constexpr int RW_OVER = 512;
 constexpr int RW_STAGE = (int)REC_SEG + RW_OVER + 16;
 
 struct StagedBytes {
-    const uint8_t* g;  // the data
-    const uint8_t* l;  // LDS copy of g[base, base + lim), 4-byte aligned, readable 4 bytes past lim
+    const CBM_GLOBAL uint8_t* g;  // the data
+    const CBM_LDS uint8_t* l;     // LDS copy of g[base, base + lim), 4-byte aligned, readable 4 bytes past lim
     int64_t base, lim;
     __device__ __forceinline__ uint32_t u8(int64_t o) const {
         const int64_t r = o - base;
-        return (uint64_t)r < (uint64_t)lim ? l[r] : g[o];
+        if ((uint64_t)r < (uint64_t)lim) return l[r];
+        return g[o];
     }
     // wider reads inside the stage: the two words that hold them, realigned
     __device__ __forceinline__ uint32_t u32(int64_t o) const {
         const int64_t r = o - base;
         if ((uint64_t)r + 4 <= (uint64_t)lim) {
-            const uint32_t* w = (const uint32_t*)l + (r >> 2);
+            const CBM_LDS uint32_t* w = (const CBM_LDS uint32_t*)l + (r >> 2);
             return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(r & 3));
         }
         return u8(o) | (u8(o + 1) << 8) | (u8(o + 2) << 16) | (u8(o + 3) << 24);
@@ -279,7 +280,7 @@ struct StagedBytes {
     __device__ __forceinline__ uint32_t u16(int64_t o) const {
         const int64_t r = o - base;
         if ((uint64_t)r + 2 <= (uint64_t)lim) {
-            const uint32_t* w = (const uint32_t*)l + (r >> 2);
+            const CBM_LDS uint32_t* w = (const CBM_LDS uint32_t*)l + (r >> 2);
             return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(r & 3)) & 0xffffu;
         }
         return u8(o) | (u8(o + 1) << 8);
@@ -366,7 +367,7 @@ __global__ void __launch_bounds__(RW_WAVES * CBM_WAVE)
     for (int i = nvec * 16 + lane; i < lim; i += CBM_WAVE) l[i] = b[abase + i];
     __syncthreads();
     if (!have) return;
-    const StagedBytes a{b, l, abase, lim};
+    const StagedBytes a{(const CBM_GLOBAL uint8_t*)b, (const CBM_LDS uint8_t*)l, abase, lim};
     int64_t f = -1;
     if (first) {
         f = lo;
@@ -593,10 +594,20 @@ __device__ __forceinline__ void csr_fill_words(uint32_t* out, const int64_t* oof
                                                const int64_t* sh_off, const int64_t* sh_src, const uint8_t* st,
                                                int64_t a0, int64_t hi, const uint8_t* b) {
     const int64_t base = ooff[i0], total = ooff[i0 + m] - base;
+    const CBM_LDS uint32_t* sw = (const CBM_LDS uint32_t*)st;
+    const CBM_GLOBAL uint8_t* bg = (const CBM_GLOBAL uint8_t*)b;
     for (int64_t j = threadIdx.x; j < total; j += FIELDS_THREADS) {
         const int r = csr_record(sh_off, m, j);
         const int64_t src = sh_src[r] + 4 * (j - sh_off[r]);
-        out[base + j] = (src >= a0 && src + 4 <= hi) ? ld_u32(st, src - a0) : ld_u32(b, src);
+        uint32_t v;
+        if (src >= a0 && src + 4 <= hi) {
+            const int x = (int)(src - a0);
+            v = __builtin_amdgcn_alignbyte(sw[(x >> 2) + 1], sw[x >> 2], (uint32_t)(x & 3));
+        } else {
+            v = (uint32_t)bg[src] | ((uint32_t)bg[src + 1] << 8) | ((uint32_t)bg[src + 2] << 16) |
+                ((uint32_t)bg[src + 3] << 24);
+        }
+        out[base + j] = v;
     }
 }
 
@@ -627,7 +638,8 @@ __device__ __forceinline__ void csr_fill_bytes(uint8_t* out, const int64_t* ooff
                     src = sh_src[r];
                     nxt = r + 1 < m ? sh_off[r + 1] + base : end;
                 }
-                x = (src >= a0 && src < hi) ? st[src - a0] : b[src];
+                if (src >= a0 && src < hi) x = ((const CBM_LDS uint8_t*)st)[src - a0];
+                else x = ((const CBM_GLOBAL uint8_t*)b)[src];
                 src++;
             }
             v[k] = x;
