@@ -537,10 +537,18 @@ __device__ __forceinline__ void parse_record(const uint8_t* p, int64_t o, int64_
 // so the workgroup's records are one byte range: it is staged in LDS with coalesced 16-byte
 // loads, and each thread decodes its record from there (one thread per record reading the
 // data in place would touch 64 cache lines per load instruction of a wavefront). A record
-// outside the staged range (offsets that are not consecutive records, or a range larger than
-// the stage) is decoded from global memory.
-constexpr int SC_STAGE = 40960;
-constexpr int FIELDS_NT = 128;  // records per workgroup: 128 GEX-like records (~250 B) fit the stage
+// outside the staged range (offsets that are not consecutive records, or past the stage's end
+// when the range is longer than the stage) is decoded from global memory.
+// 24 KiB: 128 records of up to 190 bytes fit, and six workgroups share a CU's LDS (a 40 KiB stage,
+// three workgroups, made the columns 1.45x slower); a longer range is staged in part
+#ifndef CBM_SC_STAGE
+#define CBM_SC_STAGE 24576
+#endif
+constexpr int SC_STAGE = CBM_SC_STAGE;
+#ifndef CBM_FIELDS_NT
+#define CBM_FIELDS_NT 128
+#endif
+constexpr int FIELDS_NT = CBM_FIELDS_NT;  // records per workgroup
 constexpr int FIELDS_THREADS = 256;  // threads per workgroup (all stage and copy; FIELDS_NT decode)
 
 // Stage data bytes [a0, a0 + nb) (a0 16-byte aligned in memory, nb <= SC_STAGE) into st with
@@ -670,7 +678,8 @@ __global__ void __launch_bounds__(FIELDS_THREADS) rec_columns(const uint8_t* b, 
         const int64_t hi = last + 4 + (int64_t)ld_u32(b, last);
         const int64_t a0 = lo - (int64_t)(((uintptr_t)(b + lo)) & 15);
         range[0] = a0;
-        range[1] = (last >= lo && hi - a0 <= SC_STAGE) ? hi : a0;
+        // as much of the range as the stage holds; records past it are read in place
+        range[1] = last >= lo ? (hi - a0 <= SC_STAGE ? hi : a0 + SC_STAGE) : a0;
     }
     __syncthreads();
     const int64_t a0 = range[0], hi = range[1];
