@@ -5,25 +5,26 @@
 // decoder started at an arbitrary bit soon falls onto the true symbol boundaries
 // (the speculative self-synchronising decode of Weissenberger and Schmidt,
 // "Massively Parallel Huffman Decoding on GPUs", ICPP 2018). Each round of a
-// Huffman block gives thread t the SEG bits that start at base + t * SEG.
-// Pass A decodes speculatively, from SEG bits before the segment (so that the walk
-// is most likely on the true boundaries when it enters it) to the first symbol
+// Huffman block gives thread t the SEG bits that start at base + t * SEG (fewer
+// in the last round of the final Huffman block, where the payload's end bounds it).
+// Pass A decodes speculatively, from 1.5 * SEG bits before the segment (so that the
+// walk is most likely on the true boundaries when it enters it) to the first symbol
 // that starts at or after the next segment, and keeps checkpoints inside its
 // segment: the symbol start before every few symbols, with the output bytes and
-// matches counted up to there. The walks are then linked: thread
-// t's true entry is thread t-1's exit; a thread whose entry changed decodes again
-// from it until it lands on a checkpoint (from there its first walk was right, and
-// its count from there is known) - repeated until nothing changes. A workgroup
-// prefix sum of the walks' counts places them, and pass C writes the literals into
-// the LDS output window and the matches, in output order, into a per-workgroup
-// list in global scratch. Pass D
-// applies the matches. Variants whose staging area holds a bit per output byte
-// keep the round's pending bytes in a bitmap there and apply, step by step, every
-// match whose source bytes are no longer pending; the others go frontier by
-// frontier (with F the destination of the first match not yet applied, every
-// byte before F is final).
+// matches counted up to there. The walks are then linked: thread t's true entry
+// is thread t-1's exit; a thread whose entry changed decodes again from it until
+// it lands on a checkpoint (from there its first walk was right, and its count
+// from there is known) - repeated until nothing changes, within each wavefront
+// by lane shifts and across wavefronts through LDS. A workgroup prefix sum of the
+// walks' counts places them, and pass C writes the literals into the LDS output
+// window and the matches, in output order, into a per-workgroup list in global
+// scratch. Pass D applies the matches. Variants whose staging area holds a bit per
+// output byte keep the round's pending bytes in a bitmap there and apply, step by
+// step, every match whose source bytes are no longer pending; the others go
+// frontier by frontier (with F the destination of the first match not yet
+// applied, every byte before F is final).
 // A dynamic block's code lengths are decoded the same way by the 64 lanes of one
-// wavefront, 32 bits a lane.
+// wavefront, CL_SEG bits a lane.
 // The block's output then leaves LDS in one coalesced store, and its CRC32 is
 // computed from LDS by all threads (slices combined with x^(8n) mod P, the
 // method of zlib's crc32_combine).
