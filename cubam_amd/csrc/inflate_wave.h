@@ -48,7 +48,6 @@ template <int NT, int SEG>
 struct WaveSmem {
     // staged compressed bits of a round; at least 1 << LBITS words, the table build's scratch
     static constexpr int IN_WORDS = NT * SEG / 32 + 8 > (1 << LBITS) + 8 ? NT * SEG / 32 + 8 : (1 << LBITS) + 8;
-    static_assert(IN_WORDS >= 256, "the staging area holds the CRC32 byte table at block end");
     static_assert(IN_WORDS >= (1 << LBITS), "the staging area is the table build's scratch");
     // the tables and the staging area first: their offsets fit the 16-bit offset field of the
     // LDS instructions, so a lookup needs no address add
@@ -553,6 +552,26 @@ __device__ void stage(S& s, const uint8_t* in, int w0, int limit) {
         s.in[i] = w;
     }
 }
+
+// ------------------------------------------------------------------ CRC32 tables
+// Slicing-by-4 tables (Kounavis and Berry, 2008): table k maps a byte followed by k zero bytes,
+// for the reflected polynomial 0xEDB88320. Built at compile time; read through the vector memory
+// path (they stay in the L1 cache), which leaves the LDS, shared by every phase, to the decoders.
+struct CrcTables {
+    uint32_t t[4][256];
+};
+constexpr CrcTables make_crc_tables() {
+    CrcTables c{};
+    for (int i = 0; i < 256; i++) {
+        uint32_t x = (uint32_t)i;
+        for (int k = 0; k < 8; k++) x = (x & 1) ? 0xEDB88320u ^ (x >> 1) : x >> 1;
+        c.t[0][i] = x;
+    }
+    for (int k = 1; k < 4; k++)
+        for (int i = 0; i < 256; i++) c.t[k][i] = (c.t[k - 1][i] >> 8) ^ c.t[0][c.t[k - 1][i] & 255];
+    return c;
+}
+__constant__ CrcTables kCrcTab = make_crc_tables();
 
 // ------------------------------------------------------------------ the kernel
 // two workgroups share a CU (the LDS holds two output windows): ask for the registers that let
@@ -1296,18 +1315,10 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             }
             for (int i = head + 16 * nvec + tid; i < nout; i += NT) d[i] = s.out[i];
         }
-        // CRC32 of the output: slice tid covers [nout - SLICE*(NT - tid), nout - SLICE*(NT-1-tid)).
-        // The tables go in the staging area, free once the last round is done: with room for
-        // four, slicing-by-4 (Kounavis and Berry, 2008: table k maps a byte followed by k zero
-        // bytes), four bytes a step; else a byte a step.
-        constexpr int CRC_TABS = S::IN_WORDS >= 1024 ? 4 : 1;
-        uint32_t* crcT = s.in;
-        for (int i = tid; i < 256 * CRC_TABS; i += NT) {
-            uint32_t t = i & 255;
-            for (int k = 0; k < 8 * (1 + (i >> 8)); k++) t = (t & 1) ? 0xEDB88320u ^ (t >> 1) : t >> 1;
-            crcT[i] = t;
-        }
-        __syncthreads();
+        // CRC32 of the output: slice tid covers [nout - SLICE*(NT - tid), nout - SLICE*(NT-1-tid)),
+        // slicing-by-4 with the tables in global memory (kCrcTab, L1-resident): four bytes a step,
+        // read from LDS as aligned words realigned by the slice's offset
+        const uint32_t(*crcG)[256] = kCrcTab.t;
         uint32_t c = 0;
         {
             int e2 = nout - SLICE * (NT - 1 - tid), s2 = e2 - SLICE;
@@ -1315,20 +1326,16 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             if (e2 > s2) {
                 uint32_t x = 0xFFFFFFFFu;
                 int k = s2;
-                if constexpr (CRC_TABS == 4) {
-                    // four bytes a step, read as aligned words realigned by the slice's offset
-                    const uint32_t* ow = (const uint32_t*)s.out;
-                    const uint32_t sh = (uint32_t)s2 & 3;
-                    uint32_t lo = ow[s2 >> 2];
-                    for (; k + 4 <= e2; k += 4) {
-                        const uint32_t hi = ow[(k >> 2) + 1];
-                        x ^= __builtin_amdgcn_alignbyte(hi, lo, sh);
-                        lo = hi;
-                        x = crcT[768 + (x & 255)] ^ crcT[512 + ((x >> 8) & 255)] ^ crcT[256 + ((x >> 16) & 255)] ^
-                            crcT[x >> 24];
-                    }
+                const uint32_t* ow = (const uint32_t*)s.out;
+                const uint32_t sh = (uint32_t)s2 & 3;
+                uint32_t lo = ow[s2 >> 2];
+                for (; k + 4 <= e2; k += 4) {
+                    const uint32_t hi = ow[(k >> 2) + 1];
+                    x ^= __builtin_amdgcn_alignbyte(hi, lo, sh);
+                    lo = hi;
+                    x = crcG[3][x & 255] ^ crcG[2][(x >> 8) & 255] ^ crcG[1][(x >> 16) & 255] ^ crcG[0][x >> 24];
                 }
-                for (; k < e2; k++) x = crcT[(x ^ s.out[k]) & 255] ^ (x >> 8);
+                for (; k < e2; k++) x = crcG[0][(x ^ s.out[k]) & 255] ^ (x >> 8);
                 c = crc_mulmod(xfac, x ^ 0xFFFFFFFFu);
             }
         }
