@@ -88,6 +88,7 @@ __global__ void rec_walk_one(const uint8_t* b, int64_t start, int64_t n, int n_r
     (void)lo;
     WalkOut r = walk(b, n, entry, hi, n_ref);
     w.f[k] = entry;
+    w.nrel[k] = -1;  // its starts are not recorded: rec_emit walks it
     w.e[k] = r.e;
     w.cnt[k] = r.cnt;
     w.cig[k] = r.cig;

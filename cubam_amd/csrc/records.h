@@ -37,9 +37,15 @@ struct RecWs {
     int64_t* base;  // exclusive scan of cnt [nseg + 1]
     int64_t* pred;  // walks that land on each segment
     int64_t* hdr;   // [0] bad links [1..7] spare
+    int64_t* nrel;  // record starts the walk recorded in rel, or -1 (rec_emit walks the segment again)
+    uint16_t* rel;  // [nseg][REC_MAXR]: each record start of the walk, less its first
     void* scan_ws;
     int64_t nseg;
 };
+
+// record starts a segment's walk records: a record takes at least 4 + 32 + 1 bytes, so at most
+// REC_SEG / 37 + 1 of them start in a segment
+constexpr int REC_MAXR = (int)(REC_SEG / 37 + 2);
 
 inline int64_t rec_nseg(int64_t len) { return (len + REC_SEG - 1) / REC_SEG; }
 
@@ -58,13 +64,17 @@ inline RecWs rec_ws_layout_seg(void* ws, int64_t nseg) {
     w.flag = p; p += nseg;
     w.pred = p; p += nseg;
     w.base = p; p += nseg + 1;
+    w.nrel = p; p += nseg;
+    w.rel = (uint16_t*)p; p += (nseg * REC_MAXR + 3) / 4;
     w.scan_ws = p;
     return w;
 }
 
 inline RecWs rec_ws_layout(void* ws, int64_t len) { return rec_ws_layout_seg(ws, rec_nseg(len)); }
 
-inline int64_t rec_ws_bytes_seg(int64_t nseg) { return 8 * (8 + 9 * nseg + 1) + scan_workspace(nseg) + 64; }
+inline int64_t rec_ws_bytes_seg(int64_t nseg) {
+    return 8 * (8 + 10 * nseg + 1) + 8 * ((nseg * REC_MAXR + 3) / 4) + scan_workspace(nseg) + 64;
+}
 
 inline int64_t rec_ws_bytes(int64_t len) { return rec_ws_bytes_seg(rec_nseg(len)); }
 
@@ -173,6 +183,11 @@ __global__ void __launch_bounds__(REC_NT) rec_emit(const uint8_t* b, int64_t sta
     int64_t o = w.f[k];
     if (o < 0) return;
     int64_t c = w.cnt[k], idx = w.base[k];
+    if (w.nrel[k] == c) {  // the walk recorded every start: no chain of dependent loads
+        const uint16_t* rel = w.rel + k * REC_MAXR;
+        for (int64_t i = 0; i < c && idx + i < n_rec; i++) off[idx + i] = o + rel[i];
+        return;
+    }
     for (int64_t i = 0; i < c && idx + i < n_rec; i++) {
         off[idx + i] = o;
         o += 4 + (int64_t)ld_u32(b, o);
@@ -296,8 +311,9 @@ struct StagedBytes {
 // through block_size is followed 64 records at a time (the same by every lane; lane j keeps the
 // j-th offset), then each lane tests its record; the first that fails ends the walk there.
 template <class A>
-__device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane) {
+__device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane, uint16_t* rel) {
     WalkOut w = {o, 0, 0, 0, 0, 0};
+    const int64_t o0 = o;
     while (o < hi && !w.flag) {
         int64_t mine = -1;
         int nrec = 0;
@@ -313,6 +329,7 @@ __device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n
         const uint64_t badm = __ballot(bad);
         const int good = badm ? __ffsll((unsigned long long)badm) - 1 : nrec;
         int64_t cig = 0, seqb = 0, qual = 0;
+        if (lane < good && w.cnt + lane < REC_MAXR) rel[w.cnt + lane] = (uint16_t)(mine - o0);
         if (lane < good) {
             const uint32_t lseq = a.u32(mine + 20);
             cig = a.u16(mine + 16);
@@ -387,9 +404,10 @@ __global__ void __launch_bounds__(RW_WAVES * CBM_WAVE)
         }
     }
     WalkOut r = {-1, 0, 0, 0, 0, 0};
-    if (f >= 0) r = walk_wave(a, n, f, hi, n_ref, lane);
+    if (f >= 0) r = walk_wave(a, n, f, hi, n_ref, lane, w.rel + k * REC_MAXR);
     if (lane == 0) {
         w.f[k] = f;
+        w.nrel[k] = r.cnt <= REC_MAXR ? r.cnt : -1;
         w.pred[k] = 0;
         w.e[k] = r.e;
         w.cnt[k] = r.cnt;
