@@ -418,7 +418,7 @@ int cbm_record_offsets(const uint8_t* data, int64_t start, int64_t n, const void
     if (n_rec == 0) return CBM_OK;
     ARG(data && offsets, "cbm_record_offsets: null pointer");
     RecWs w = rec_ws_layout((void*)ws, n - start);
-    unsigned g = (unsigned)((w.nseg + REC_NT - 1) / REC_NT);
+    unsigned g = (unsigned)((w.nseg + EMIT_SEGS - 1) / EMIT_SEGS);
     hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, start, n, w, offsets, n_rec);
     return check_launch("rec_emit");
 }
@@ -511,7 +511,7 @@ int cbm_record_offsets_spans(const uint8_t* data, int64_t n_seg, const void* ws,
     if (n_rec == 0 || n_seg == 0) return CBM_OK;
     ARG(data && offsets, "cbm_record_offsets_spans: null pointer");
     RecWs w = rec_ws_layout_seg((void*)ws, n_seg);
-    unsigned g = (unsigned)((n_seg + REC_NT - 1) / REC_NT);
+    unsigned g = (unsigned)((n_seg + EMIT_SEGS - 1) / EMIT_SEGS);
     hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, (int64_t)0, (int64_t)0, w,
                        offsets, n_rec);
     return check_launch("rec_emit");

@@ -176,18 +176,23 @@ __global__ void rec_result_init(int64_t* result, int64_t start, RecWs w) {
 __global__ void rec_result_links(int64_t* result, RecWs w) { result[CBM_REC_BAD_LINKS] = w.hdr[0]; }
 
 // offsets of the records of each walk, at the walk's base
+// a wavefront per segment: lanes copy its recorded starts, coalesced; a segment whose starts were
+// not recorded is walked by lane 0
+constexpr int EMIT_SEGS = REC_NT / CBM_WAVE;  // segments per workgroup
 __global__ void __launch_bounds__(REC_NT) rec_emit(const uint8_t* b, int64_t start, int64_t n, RecWs w,
                                                    int64_t* off, int64_t n_rec) {
-    int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
+    const int lane = (int)threadIdx.x % CBM_WAVE;
+    const int64_t k = (int64_t)blockIdx.x * EMIT_SEGS + (int64_t)threadIdx.x / CBM_WAVE;
     if (k >= w.nseg) return;
     int64_t o = w.f[k];
     if (o < 0) return;
-    int64_t c = w.cnt[k], idx = w.base[k];
+    const int64_t c = w.cnt[k], idx = w.base[k];
     if (w.nrel[k] == c) {  // the walk recorded every start: no chain of dependent loads
         const uint16_t* rel = w.rel + k * REC_MAXR;
-        for (int64_t i = 0; i < c && idx + i < n_rec; i++) off[idx + i] = o + rel[i];
+        for (int64_t i = lane; i < c && idx + i < n_rec; i += CBM_WAVE) off[idx + i] = o + rel[i];
         return;
     }
+    if (lane) return;
     for (int64_t i = 0; i < c && idx + i < n_rec; i++) {
         off[idx + i] = o;
         o += 4 + (int64_t)ld_u32(b, o);
