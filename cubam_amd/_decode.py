@@ -95,6 +95,7 @@ class Blocks:
     isize: torch.Tensor
     crc: torch.Tensor
     out_off: torch.Tensor  # n + 1 entries
+    n_cand: int = 0  # header candidates the scan found (more than n_blocks: some were not on the chain)
 
 
 def scan_blocks(buf: torch.Tensor, n: int, eof: bool, ws: Workspace | None = None) -> Blocks:
@@ -136,7 +137,7 @@ def scan_blocks(buf: torch.Tensor, n: int, eof: bool, ws: Workspace | None = Non
         raise ValueError(MSG_CORRUPT)
     nb = int(h[_lib.SCAN_NBLOCKS])
     return Blocks(buf, nb, int(h[_lib.SCAN_USED]), int(h[_lib.SCAN_TOTAL_OUT]), blk_off[:nb], bsize[:nb],
-                  isize[:nb], crc[:nb], out_off[: nb + 1])
+                  isize[:nb], crc[:nb], out_off[: nb + 1], int(h[_lib.SCAN_NCAND]))
 
 
 _tune = _lib.InflateTune(0, 0, 0, 0)
@@ -210,16 +211,24 @@ def check_inflated(inf: Inflated) -> None:
 @dataclass
 class Chain:
     n_rec: int
-    tail: int
+    tail: int  # where the chain stopped: the end of the data, an incomplete record or an invalid one
     n_cigar: int
     n_seq: int
     n_qual: int
     ws: torch.Tensor
+    invalid: bool = False  # the chain stopped at a record that fails the structural test
+    bad_links: int = 0  # segments whose speculative walks did not link (repaired before returning)
 
 
 def record_chain(data: torch.Tensor, start: int, n: int, n_ref: int, ws: Workspace | None = None,
-                 inf: Inflated | None = None) -> Chain:
-    """Kernel 3: the record chain of data[start:n) (synchronises once; raises on corrupt data)."""
+                 inf: Inflated | None = None, strict: bool = True) -> Chain:
+    """Kernel 3: the record chain of data[start:n) (synchronises once).
+
+    A record that fails the structural test ends the chain there. With ``strict`` that raises
+    (a region span must hold whole records); otherwise the chain up to it is returned with
+    ``invalid`` set, and ``tail`` is where the invalid record starts, as the reference's
+    pointer-jumping chain keeps the records before it (src/cubam/_kernels.py:676-702).
+    """
     L = _lib.load()
     ws = ws or default_workspace()
     st = stream_ptr()
@@ -235,14 +244,29 @@ def record_chain(data: torch.Tensor, start: int, n: int, n_ref: int, ws: Workspa
     torch.cuda.current_stream().synchronize()
     if inf is not None:
         raise_block_errors(inf, int(h[8]))
-    if int(h[_lib.REC_BAD_LINKS]):
+    bad_links = int(h[_lib.REC_BAD_LINKS])
+    if bad_links:
         check(L.cbm_record_chain_repair(ptr(data), start, n, n_ref, ptr(result), ptr(rws), rws.numel(), st),
               "cbm_record_chain_repair")
         h[:8].copy_(result[:64].view(_I64))
-    if int(h[_lib.REC_INVALID]):
+    invalid = bool(int(h[_lib.REC_INVALID]))
+    if invalid and strict:
         raise ValueError(MSG_NO_BLOCK)
     return Chain(int(h[_lib.REC_N]), int(h[_lib.REC_TAIL]), int(h[_lib.REC_CIGAR]), int(h[_lib.REC_SEQ]),
-                 int(h[_lib.REC_QUAL]), rws)
+                 int(h[_lib.REC_QUAL]), rws, invalid, bad_links)
+
+
+def record_candidates(data: torch.Tensor, start: int, n: int, n_ref: int, ws: Workspace | None = None) -> int:
+    """Byte offsets of data[start:n) where a valid record starts (synchronises).
+
+    The reference's record_candidates count (src/cubam/_kernels.py:280-284, 662-673); the
+    reader needs it only when the chain from ``start`` holds no record.
+    """
+    ws = ws or default_workspace()
+    cnt = ws.dev("rec_cand", 64)
+    check(_lib.load().cbm_record_candidates(ptr(data), start, n, n_ref, ptr(cnt), stream_ptr()),
+          "cbm_record_candidates")
+    return int(cnt[:8].view(_I64)[0].item())
 
 
 def record_offsets(data: torch.Tensor, start: int, n: int, chain: Chain) -> torch.Tensor:

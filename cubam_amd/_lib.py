@@ -78,6 +78,7 @@ SYMBOLS = (
     "cbm_record_chain_workspace",
     "cbm_record_chain",
     "cbm_record_chain_repair",
+    "cbm_record_candidates",
     "cbm_record_offsets",
     "cbm_record_spans_workspace",
     "cbm_record_chain_spans",
@@ -129,6 +130,8 @@ def load():
     L.cbm_record_chain.argtypes = [vp, i64, i64, i32, vp, vp, i64, vp]
     L.cbm_record_chain_repair.restype = ctypes.c_int
     L.cbm_record_chain_repair.argtypes = [vp, i64, i64, i32, vp, vp, i64, vp]
+    L.cbm_record_candidates.restype = ctypes.c_int
+    L.cbm_record_candidates.argtypes = [vp, i64, i64, i32, vp, vp]
     L.cbm_record_offsets.restype = ctypes.c_int
     L.cbm_record_offsets.argtypes = [vp, i64, i64, vp, i64, vp, i64, vp]
     L.cbm_record_spans_workspace.restype = i64

@@ -3,8 +3,8 @@
 :func:`inflate_chunks` reads a file in chunks and yields the inflated data of
 each chunk as a device ``torch.uint8`` tensor:
 
-1. **Read.** A background thread reads the next chunks into pinned host buffers
-   with parallel ``pread`` calls (as the reference's ``_ChunkReader``).
+1. **Read.** A thread pool reads the next chunks into pinned host slots with
+   parallel ``preadv`` calls, a slot a chunk, as slots are handed back.
 2. **Find blocks.** Kernel 1 finds the chain of BGZF blocks from the start of
    the chunk; the bytes of an incomplete last block stay on the GPU and go in
    front of the next chunk.
@@ -15,9 +15,9 @@ each chunk as a device ``torch.uint8`` tensor:
 from __future__ import annotations
 
 import os
-import queue
-import threading
+from collections import deque
 from collections.abc import Callable, Iterator
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -33,85 +33,80 @@ BLOCK_DATA = 0xFF00
 
 
 class _ChunkReader:
-    """Chunks of a file read ahead into pinned buffers by a background thread.
+    """Compressed chunks of a file, read ahead into pinned host slots (the reference's
+    ``_ChunkReader`` contract, src/cubam/bgzf.py:38-125: slots, ``host``/``release``,
+    ``n_buffers``, ``n_threads``, ``piece``).
 
-    Each chunk is read by ``n_threads`` parallel ``pread`` calls of ``piece``
-    bytes. Iterating yields ``(slot, nbytes, eof)``; the consumer copies
-    :meth:`host` ``(slot)[:nbytes]`` and then calls :meth:`release` ``(slot)``.
-    Same contract as the reference (src/cubam/bgzf.py:38-125).
+    Chunk k is the file's bytes [k * chunk_bytes, (k + 1) * chunk_bytes). Its read is
+    split into ``piece``-byte ``preadv`` calls that a pool of ``n_threads`` threads runs;
+    up to ``n_buffers`` chunks are in flight, one a slot. Iterating yields
+    ``(slot, nbytes, eof)`` in file order once a chunk is whole; ``release(slot)``
+    hands the slot back and starts the read of the next unread chunk into it. A
+    failed read raises when its chunk is reached.
     """
 
     def __init__(self, path, chunk_bytes: int, *, n_buffers: int = 3, n_threads: int = 8,
                  piece: int = 16 << 20):
-        self.path, self.chunk, self.n_threads, self.piece = Path(path), chunk_bytes, n_threads, piece
-        self._mem = [torch.empty(chunk_bytes, dtype=torch.uint8, pin_memory=True) for _ in range(n_buffers)]
-        self._free: queue.Queue = queue.Queue()
-        for i in range(n_buffers):
-            self._free.put(i)
-        self._full: queue.Queue = queue.Queue()
-        self._error: BaseException | None = None
-        self._stop = False
-        self._thread = threading.Thread(target=self._run, daemon=True)
-        self._thread.start()
+        if chunk_bytes <= 0 or piece <= 0 or n_buffers < 1 or n_threads < 1:
+            raise ValueError("chunk_bytes, piece, n_buffers and n_threads must be positive")
+        self.path, self.chunk, self.piece = Path(path), chunk_bytes, piece
+        self._fd = os.open(self.path, os.O_RDONLY)
+        self._size = os.fstat(self._fd).st_size
+        self._last = max(0, (self._size - 1) // chunk_bytes)  # index of the final chunk (an empty file has one)
+        slot_bytes = max(1, min(chunk_bytes, self._size))
+        n_slots = min(n_buffers, self._last + 1)
+        pin = torch.cuda.is_available()  # page-locked for the device copies (plain memory where no GPU is)
+        self._mem = [torch.empty(slot_bytes, dtype=torch.uint8, pin_memory=pin) for _ in range(n_slots)]
+        self._pool = ThreadPoolExecutor(n_threads)
+        self._queue: deque = deque()  # (chunk index, slot, nbytes, futures) in file order
+        self._next = 0
+        self._closed = False
+        for slot in range(n_slots):
+            self._start(slot)
+
+    @property
+    def slots(self) -> int:
+        return len(self._mem)
 
     def host(self, slot: int) -> torch.Tensor:
         return self._mem[slot]
 
     def release(self, slot: int) -> None:
-        self._free.put(slot)
+        if not self._closed and self._next <= self._last:
+            self._start(slot)
 
     def close(self) -> None:
-        self._stop = True
-        self._free.put(-1)
+        if self._closed:
+            return
+        self._closed = True
+        self._pool.shutdown(wait=True, cancel_futures=True)
+        os.close(self._fd)
 
-    @staticmethod
-    def _pread(fd: int, mv: memoryview, offset: int) -> None:
-        while len(mv):
-            got = os.preadv(fd, [mv], offset)
-            if not got:
+    def _start(self, slot: int) -> None:
+        k = self._next
+        self._next += 1
+        lo = k * self.chunk
+        n = max(0, min(self.chunk, self._size - lo))
+        view = memoryview(self._mem[slot].numpy())
+        futures = [self._pool.submit(self._fill, view[p : min(p + self.piece, n)], lo + p)
+                   for p in range(0, n, self.piece)]
+        self._queue.append((k, slot, n, futures))
+
+    def _fill(self, dst: memoryview, at: int) -> None:
+        while dst.nbytes:
+            got = os.preadv(self._fd, [dst], at)
+            if got <= 0:
                 raise ValueError("The file is shorter than its size")
-            mv, offset = mv[got:], offset + got
-
-    def _run(self) -> None:
-        from concurrent.futures import ThreadPoolExecutor
-
-        try:
-            fd = os.open(self.path, os.O_RDONLY)
-            try:
-                size = os.fstat(fd).st_size
-                off = 0
-                with ThreadPoolExecutor(self.n_threads) as ex:
-                    while True:
-                        slot = self._free.get()
-                        if self._stop:
-                            return
-                        n = min(self.chunk, size - off)
-                        mv = memoryview(self._mem[slot].numpy())
-                        futures = [
-                            ex.submit(self._pread, fd, mv[p : min(p + self.piece, n)], off + p)
-                            for p in range(0, n, self.piece)
-                        ]
-                        for f in futures:
-                            f.result()
-                        off += n
-                        eof = off >= size
-                        self._full.put((slot, n, eof))
-                        if eof:
-                            return
-            finally:
-                os.close(fd)
-        except BaseException as e:  # noqa: BLE001  (re-raised in the consumer)
-            self._error = e
-            self._full.put(None)
+            dst, at = dst[got:], at + got
 
     def __iter__(self) -> Iterator[tuple[int, int, bool]]:
-        while True:
-            item = self._full.get()
-            if item is None:
-                raise self._error  # type: ignore[misc]
-            yield item
-            if item[2]:
-                return
+        while self._next <= self._last or self._queue:
+            if not self._queue:
+                raise RuntimeError("_ChunkReader: every slot is held; release one before reading on")
+            k, slot, n, futures = self._queue.popleft()
+            for f in futures:
+                f.result()
+            yield slot, n, k == self._last
 
 
 def _inflate_blocks(buf: torch.Tensor, n: int, eof: bool, prefix: torch.Tensor | None = None,
@@ -159,8 +154,8 @@ def _inflate(path, chunk_bytes: int | None, prefix: Callable[[], torch.Tensor] |
 
     As the reference's ``_inflate`` (src/cubam/bgzf.py:625-652): the compressed
     carry (the incomplete last block) stays on the device in front of the next
-    chunk; ``prefix`` is called before each chunk is inflated. The reader thread
-    reads chunks ahead into pinned buffers; each chunk's host-to-device copy runs
+    chunk; ``prefix`` is called before each chunk is inflated. The reader's pool
+    reads chunks ahead into pinned slots; each chunk's host-to-device copy runs
     on a copy stream while the chunk before it is decoded, and the carry and the
     uploaded chunk are joined on the device.
     """
@@ -170,7 +165,8 @@ def _inflate(path, chunk_bytes: int | None, prefix: Callable[[], torch.Tensor] |
     chunk_bytes = chunk_bytes or auto_chunk_bytes()
     reader = _ChunkReader(path, chunk_bytes, **reader_kw)
     compute = torch.cuda.current_stream()
-    copier = torch.cuda.Stream() if OVERLAP_UPLOAD else compute
+    overlap = OVERLAP_UPLOAD and reader.slots > 1  # with one slot the next chunk waits for its release
+    copier = torch.cuda.Stream() if overlap else compute
     items = iter(reader)
     carry = torch.empty(0, dtype=torch.uint8, device=dev)
 
@@ -198,12 +194,12 @@ def _inflate(path, chunk_bytes: int | None, prefix: Callable[[], torch.Tensor] |
                 buf[:c].copy_(carry)
             buf[c : c + nbytes].copy_(stage)
             del stage
-            if OVERLAP_UPLOAD:
+            if overlap:
                 pending = None if eof else upload()  # the next chunk's copy overlaps this chunk's decode
             data, n, used, inf = _inflate_blocks(buf, c + nbytes, eof, prefix() if prefix else None, ws)
-            if not OVERLAP_UPLOAD:
-                pending = None if eof else upload()
             reader.release(slot)  # the scan synchronised the compute stream, which waited for the copy
+            if not overlap:
+                pending = None if eof else upload()
             carry = buf[used : c + nbytes].clone()
             del buf
             yield data, n, inf

@@ -108,6 +108,18 @@ __global__ void rec_pred_fill(RecWs w) {
         w.pred[k] = w.f[k] >= 0 ? 1 : 0;
 }
 
+// Offsets of data[start, n) that pass is_record, counted (a thread an offset, grid-stride). The
+// structural test reads at most 3 bytes past a record that fits, which CBM_PAD keeps readable.
+__global__ void __launch_bounds__(256) rec_candidates_count(const uint8_t* b, int64_t start, int64_t n, int n_ref,
+                                                            unsigned long long* count) {
+    unsigned long long c = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t o = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += stride)
+        c += is_record(b, n, o, n_ref) ? 1ull : 0ull;
+    c = wave_sum(c);
+    if (lane_id() == 0 && c) atomicAdd(count, c);
+}
+
 // Host side of the chain repair: walk each span's chain from its start, re-walking every
 // segment whose speculative entry is not the true exit of the walk before it.
 static int repair_load(const RecWs& w, std::vector<int64_t>& f, std::vector<int64_t>& e, std::vector<int64_t>& flag,
@@ -409,6 +421,21 @@ int cbm_record_chain_repair(const uint8_t* data, int64_t start, int64_t n, int32
                         (int64_t*)nullptr, st);
     if (rc) return rc;
     return hipStreamSynchronize(st) == hipSuccess ? CBM_OK : CBM_ERR_LAUNCH;
+}
+
+int cbm_record_candidates(const uint8_t* data, int64_t start, int64_t n, int32_t n_ref, int64_t* count,
+                          void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ARG(start >= 0 && n >= start && count, "cbm_record_candidates: bad arguments");
+    int rc = fill(count, 1, 0, st);
+    if (rc || n == start) return rc;
+    ARG(data, "cbm_record_candidates: null data");
+    int64_t g = (n - start + 255) / 256;
+    const int64_t gmax = (int64_t)device_cus() * 16;
+    if (g > gmax) g = gmax;
+    hipLaunchKernelGGL(rec_candidates_count, dim3((unsigned)g), dim3(256), 0, st, data, start, n, (int)n_ref,
+                       (unsigned long long*)count);
+    return check_launch("rec_candidates_count");
 }
 
 int cbm_record_offsets(const uint8_t* data, int64_t start, int64_t n, const void* ws, int64_t ws_bytes,

@@ -22,7 +22,7 @@ import torch
 
 from cubam_amd import _decode
 from cubam_amd._lib import PAD
-from cubam_amd.bam import BamHeader
+from cubam_amd.bam import BamHeader, _Cursor
 
 
 class _ArrayFile(io.RawIOBase):
@@ -45,7 +45,7 @@ def header_of(src) -> BamHeader:
     """The header of a BAM file given as a path or a host byte array."""
     if isinstance(src, np.ndarray):
         with gzip.GzipFile(fileobj=io.BufferedReader(_ArrayFile(src))) as fh:
-            return BamHeader._parse(fh, "BAM")
+            return BamHeader._parse(_Cursor(fh.read, "BAM"))
     return BamHeader.read(src)
 
 

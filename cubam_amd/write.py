@@ -35,7 +35,8 @@ def _as_device(data) -> torch.Tensor:
         return data.reshape(-1).view(_U8) if data.dtype != _U8 else data.reshape(-1)
     if isinstance(data, (bytes, bytearray, memoryview)):
         data = np.frombuffer(bytes(data), np.uint8)
-    return torch.from_numpy(np.ascontiguousarray(data, np.uint8)).to(_decode.device())
+    # a writable copy where needed: torch does not take read-only numpy buffers
+    return torch.from_numpy(np.require(data, np.uint8, ["C", "W"])).to(_decode.device())
 
 
 def copy_spans(src: torch.Tensor, src_off: torch.Tensor, length: torch.Tensor, dst_off: torch.Tensor,
@@ -103,57 +104,69 @@ def gather_records(data: torch.Tensor, offset: torch.Tensor) -> torch.Tensor:
 
 
 class BgzfWriter:
-    """A BGZF file written from bytes on the GPU (src/cubam/bgzf.py:425-540).
+    """A BGZF file written from bytes on the GPU (the reference's BgzfWriter contract,
+    src/cubam/bgzf.py:425-540: ``write``, ``flush``, ``close``, the EOF block).
 
-    :meth:`write` appends bytes (a device tensor or host bytes); whole blocks are
-    compressed once ``flush_bytes`` are buffered. :meth:`flush` ends the current
-    block, so the next bytes start a new one. :meth:`close` compresses the rest and
-    adds the EOF block. ``path=None`` keeps the file in memory (:meth:`getvalue`).
+    Bytes written (device tensors or host bytes) go into one device staging buffer.
+    When it holds ``flush_bytes`` or is full, its whole 65,280-byte blocks are
+    compressed in one launch and written out, and the partial block left over moves to
+    the front. :meth:`flush` compresses everything staged, so the next byte starts a
+    block; :meth:`close` does that and appends the EOF block. Block boundaries thus fall
+    every 65,280 bytes from the last :meth:`flush`, whatever the sizes of the writes.
+    ``path=None`` keeps the file in memory (:meth:`getvalue`).
     """
 
     def __init__(self, path: str | Path | None, flush_bytes: int = 1024 * BLOCK_DATA):
         self.path = Path(path) if path is not None else None
-        self.flush_bytes = flush_bytes
-        self._fh = open(self.path, "wb") if self.path is not None else None  # noqa: SIM115
+        self.flush_bytes = max(int(flush_bytes), 1)
+        # whole blocks of at least flush_bytes, plus room for a partial block
+        self._cap = (self.flush_bytes + BLOCK_DATA - 1) // BLOCK_DATA * BLOCK_DATA + BLOCK_DATA
+        self._stage: torch.Tensor | None = None
+        self._fill = 0
+        self._fh = open(self.path, "wb") if self.path is not None else None  # noqa: SIM115  (closed in close())
         self._mem: list[bytes] = []
-        self._pending: list[torch.Tensor] = []
-        self._n = 0
         self.ws = _decode.Workspace()
         self.closed = False
 
-    def _emit(self, b: bytes) -> None:
-        if self._fh is not None:
-            self._fh.write(b)
-        else:
+    def _out(self, b: bytes) -> None:
+        if self._fh is None:
             self._mem.append(b)
+        else:
+            self._fh.write(b)
 
     def write(self, data) -> None:
-        d = _as_device(data)
-        if d.numel():
-            self._pending.append(d)
-            self._n += int(d.numel())
-        if self._n >= self.flush_bytes:
-            self._flush(final=False)
+        src = _as_device(data)
+        total, done = int(src.numel()), 0
+        if total and self._stage is None:
+            self._stage = torch.empty(self._cap, dtype=_U8, device=src.device)
+        while done < total:
+            take = min(total - done, self._cap - self._fill)
+            self._stage[self._fill : self._fill + take].copy_(src[done : done + take])
+            self._fill += take
+            done += take
+            if self._fill >= self.flush_bytes or self._fill == self._cap:
+                self._compress(whole_blocks=True)
 
-    def _flush(self, final: bool) -> None:
-        if not self._pending:
-            return
-        buf = torch.cat(self._pending) if len(self._pending) > 1 else self._pending[0]
-        cut = int(buf.numel()) if final else int(buf.numel()) // BLOCK_DATA * BLOCK_DATA
+    def _compress(self, whole_blocks: bool) -> None:
+        n = self._fill
+        cut = n // BLOCK_DATA * BLOCK_DATA if whole_blocks else n
         if cut:
-            self._emit(bgzf_compress(buf[:cut], ws=self.ws).cpu().numpy().tobytes())
-        rest = buf[cut:]
-        self._pending = [rest.clone()] if rest.numel() else []
-        self._n = int(rest.numel())
+            self._out(bgzf_compress(self._stage[:cut], ws=self.ws).cpu().numpy().tobytes())
+        left = n - cut
+        if left:  # the partial block moves to the front (source and target may overlap: copy it first)
+            self._stage[:left].copy_(self._stage[cut:n].clone())
+        self._fill = left
 
     def flush(self) -> None:
-        self._flush(final=True)
+        """Compress and write everything staged; the next byte written starts a new block."""
+        if self._fill:
+            self._compress(whole_blocks=False)
 
     def close(self) -> None:
         if self.closed:
             return
-        self._flush(final=True)
-        self._emit(EOF_BLOCK)
+        self.flush()
+        self._out(EOF_BLOCK)
         if self._fh is not None:
             self._fh.close()
         self.closed = True

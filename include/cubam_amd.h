@@ -169,6 +169,18 @@ int cbm_record_chain(const uint8_t* data_d, int64_t start, int64_t n, int32_t n_
 int cbm_record_chain_repair(const uint8_t* data_d, int64_t start, int64_t n, int32_t n_ref, int64_t* result_d,
                             void* ws_d, int64_t ws_bytes, void* stream);
 
+/*
+ * count_d[0] = the number of byte offsets o in [start, n) where a complete,
+ * structurally valid record starts (is_record, one thread an offset). The
+ * reader calls it only when the chain from start holds no record: the
+ * reference then raises "No valid block or record..." when any candidate
+ * exists and otherwise carries the bytes to the next chunk. Replaces the
+ * count of append_offsets("record_candidates") (src/cubam/_kernels.py:280-284,
+ * 662-673) as src/cubam/bam.py:407-412 and _kernels.py:687-689 use it.
+ */
+int cbm_record_candidates(const uint8_t* data_d, int64_t start, int64_t n, int32_t n_ref, int64_t* count_d,
+                          void* stream);
+
 /* Offsets (from data start) of the n_rec records found by cbm_record_chain, in order. */
 int cbm_record_offsets(const uint8_t* data_d, int64_t start, int64_t n, const void* ws_d, int64_t ws_bytes,
                        int64_t* offsets_d, int64_t n_rec, void* stream);

@@ -206,12 +206,13 @@ int orc_is_record(const uint8_t* b, int64_t n, int64_t o, int n_ref) {
 }
 
 /*
- * The record chain of data (starting at a record), as bam.py:403-418 keeps it
- * when every record passes is_record: offsets of the complete records and
- * where the incomplete last one starts (the tail, carried to the next chunk).
- * out[0] = records, out[1] = tail, out[2] = status: 0 ok, 2 = a complete
- * record fails is_record (the reference raises "No valid block or record..."
- * when it is the first; later ones end its chain), 4 = cap exceeded.
+ * The record chain of data (starting at a record), as bam.py:403-418 and
+ * chain (_kernels.py:676-702) keep it: from offset 0, each record that fits
+ * and passes is_record, then the next at its end. The chain stops at the
+ * first record that does not fit (the incomplete last one) or fails the test;
+ * out[1] (the tail, carried to the next chunk) is where that record starts.
+ * out[0] = records, out[2] = status: 0 ok, 2 = the chain stopped at a complete
+ * record that fails is_record, 4 = cap exceeded.
  */
 void orc_record_chain(const uint8_t* b, int64_t n, int n_ref, int64_t* off, int64_t cap, int64_t* out) {
     int64_t o = 0, k = 0;
@@ -226,6 +227,19 @@ void orc_record_chain(const uint8_t* b, int64_t n, int n_ref, int64_t* off, int6
     }
     out[0] = k;
     out[1] = o;
+}
+
+/*
+ * Offsets of b[0:n) where orc_is_record holds, counted by brute force: the
+ * count append_offsets("record_candidates") makes (_kernels.py:280-284,
+ * 662-673). bam.py:407-412 and chain (_kernels.py:687-689) use it when no
+ * record starts at offset 0: with any candidate the reference raises "No
+ * valid block or record...", with none it carries the data to the next chunk.
+ */
+int64_t orc_record_candidates(const uint8_t* b, int64_t n, int n_ref) {
+    int64_t c = 0;
+    for (int64_t o = 0; o < n; o++) c += orc_is_record(b, n, o, n_ref);
+    return c;
 }
 
 /* Sizes that the CSR columns need: n_cigar, ceil(l_seq / 2), l_seq per record. */

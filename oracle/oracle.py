@@ -39,6 +39,7 @@ MSG_CORRUPT = "Corrupt BGZF block"
 MSG_SIZE = "Corrupt BGZF block: it does not inflate to the size in its trailer"
 MSG_CRC = "Corrupt BGZF block: its CRC32 does not match the data"
 MSG_NO_BLOCK = "No valid block or record where one was expected: the data is corrupt"
+MSG_TOO_MANY = "Too many candidates: the data is corrupt"
 
 FIELDS = ("ref", "pos", "end", "mapq", "flag", "nref", "npos", "tlen", "l_seq", "clip_left", "clip_right")
 
@@ -70,6 +71,8 @@ def lib():
         L.orc_is_record.restype = i32
         L.orc_is_record.argtypes = [vp, i64, i64, i32]
         L.orc_record_chain.argtypes = [vp, i64, i32, vp, i64, vp]
+        L.orc_record_candidates.restype = i64
+        L.orc_record_candidates.argtypes = [vp, i64, i32]
         L.orc_var_sizes.argtypes = [vp, vp, i64, vp, vp]
         L.orc_decode_columns.argtypes = [vp, vp, i64] + [vp] * 17
         L.orc_find_tag.argtypes = [vp, vp, i64, ctypes.c_uint8, ctypes.c_uint8, vp, vp, vp, vp]
@@ -224,6 +227,12 @@ def record_chain(data, n_ref: int):
     return off[: int(out[0])].copy(), int(out[1]), int(out[2])
 
 
+def record_candidates(data, n_ref: int) -> int:
+    """Offsets of ``data`` where a valid record starts (brute force; _kernels.py:280-284)."""
+    b = _u8(data)
+    return int(lib().orc_record_candidates(_p(b), b.size, n_ref))
+
+
 def _excl(x: np.ndarray) -> np.ndarray:
     o = np.zeros(x.size + 1, np.int64)
     np.cumsum(x.astype(np.int64), out=o[1:])
@@ -288,26 +297,35 @@ def find_tag(data, offsets, name: str):
 
 def read_bam(file_bytes, chunk_bytes: int, tags=("CB", "UB"), n_threads: int = 1, name: str = "BAM"):
     """bam.py:421-446 on the CPU: a list of (data, columns) per chunk."""
+    return list(iter_bam(file_bytes, chunk_bytes, tags, n_threads, name))
+
+
+def iter_bam(file_bytes, chunk_bytes: int, tags=("CB", "UB"), n_threads: int = 1, name: str = "BAM"):
+    """bam.py:421-446 on the CPU, as a generator of (data, columns) per chunk: the chunks before an
+    error are yielded before it raises, as the reference's generator does."""
     fb = _u8(file_bytes)
     header = read_header(fb)
     n_ref = len(header.names)
     skip = header.size
     carry = [np.empty(0, np.uint8)]
-    out = []
     for data in _inflate_iter(fb, chunk_bytes, lambda: carry[0], n_threads):
         if skip:
             k = min(skip, data.size)
             data = data[k:]
             skip -= k
         off, tail, status = record_chain(data, n_ref)
-        if status == 2 and off.size == 0 and tail == 0:
-            raise ValueError(MSG_NO_BLOCK)
+        if off.size == 0 and data.size:
+            # no record at offset 0: bam.py:410-412 and _kernels.py:668-672, 687-689
+            cand = record_candidates(data, n_ref)
+            if cand > data.size // 32 + 1024:
+                raise ValueError(MSG_TOO_MANY)
+            if cand:
+                raise ValueError(MSG_NO_BLOCK)
         carry[0] = data[tail:].copy()
         if off.size:
-            out.append((data, columns(data, off, tags, n_threads)))
+            yield data, columns(data, off, tags, n_threads)
     if carry[0].size:
         raise ValueError(f"{name}: truncated BAM file (incomplete record at the end)")
-    return out
 
 
 def fnv1a(data) -> int:

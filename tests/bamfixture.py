@@ -48,6 +48,7 @@ class Read:
     seq: str = ""
     qual: bytes | None = None
     tags: list = field(default_factory=list)  # (name, type, value)
+    aux_raw: bytes = b""  # bytes appended after the tags (malformed aux fields)
 
     @property
     def ref_len(self) -> int:
@@ -94,8 +95,8 @@ class Read:
         out += self.qual if self.qual is not None else b"\xff" * l_seq
         for t, ty, v in self.tags:
             out += t.encode()
-            if ty == "Z":
-                out += b"Z" + v.encode() + b"\0"
+            if ty in "ZH":
+                out += ty.encode() + v.encode() + b"\0"
             elif ty == "B":
                 sub, vals = v
                 fmt = {"c": "b", "C": "B", "s": "h", "S": "H", "i": "i", "I": "I", "f": "f"}[sub]
@@ -105,6 +106,7 @@ class Read:
             else:
                 fmt = {"c": "b", "C": "B", "s": "h", "S": "H", "i": "i", "I": "I", "f": "f"}[ty]
                 out += ty.encode() + struct.pack("<" + fmt, v)
+        out += self.aux_raw
         return struct.pack("<i", len(out)) + bytes(out)
 
 

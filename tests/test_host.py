@@ -54,3 +54,97 @@ def test_write_framing_constants():
     assert _lib.DEFLATE_SLOT >= 18 + 8 + write.BLOCK_DATA + 5 * ((write.BLOCK_DATA + 65534) // 65535)
     # the EOF block is an empty stored deflate block with the CRC and size of nothing
     assert write.EOF_BLOCK[-8:] == zlib.crc32(b"").to_bytes(4, "little") + (0).to_bytes(4, "little")
+
+
+# ------------------------------------------------------------------ file reader and header (no GPU needed)
+
+
+def _read_all(path, chunk, **kw):
+    from cubam_amd.bgzf import _ChunkReader
+
+    r = _ChunkReader(path, chunk, **kw)
+    out = []
+    try:
+        for slot, n, eof in r:
+            out.append((bytes(r.host(slot)[:n].numpy()), eof))
+            r.release(slot)
+    finally:
+        r.close()
+    return out
+
+
+@pytest.mark.parametrize("size, chunk, piece, buffers, threads",
+                         [(100_000, 7_000, 1_000, 3, 4), (100_000, 7_000, 7_000, 1, 1), (70_000, 7_000, 3_001, 2, 8),
+                          (5, 1 << 20, 16 << 20, 3, 8), (0, 4096, 100, 3, 2), (4096, 4096, 4096, 2, 2)])
+def test_chunk_reader_reads_every_byte_in_order(tmp_path, size, chunk, piece, buffers, threads):
+    import numpy as np
+
+    data = np.random.default_rng(size).integers(0, 256, size, dtype=np.uint8).tobytes()
+    p = tmp_path / "f.bin"
+    p.write_bytes(data)
+    got = _read_all(p, chunk, n_buffers=buffers, n_threads=threads, piece=piece)
+    assert b"".join(g for g, _ in got) == data
+    assert [e for _, e in got] == [False] * (len(got) - 1) + [True]
+    assert all(len(g) == chunk for g, _ in got[:-1])
+    assert len(got) == max(1, -(-size // chunk))
+
+
+def test_chunk_reader_stalls_loudly_without_a_release(tmp_path):
+    from cubam_amd.bgzf import _ChunkReader
+
+    p = tmp_path / "f.bin"
+    p.write_bytes(b"x" * 10_000)
+    r = _ChunkReader(p, 1000, n_buffers=2, n_threads=2, piece=300)
+    it = iter(r)
+    next(it)
+    next(it)
+    with pytest.raises(RuntimeError, match="release"):
+        next(it)
+    r.close()
+
+
+def test_chunk_reader_reports_a_short_file(tmp_path, monkeypatch):
+    import os
+
+    from cubam_amd import bgzf
+
+    p = tmp_path / "f.bin"
+    p.write_bytes(b"y" * 5000)
+    real = os.preadv
+    monkeypatch.setattr(bgzf.os, "preadv", lambda fd, bufs, at: 0 if at >= 3000 else real(fd, bufs, at))
+    with pytest.raises(ValueError, match="shorter than its size"):
+        _read_all(p, 4000, piece=1000)
+
+
+def test_bam_header_parse_and_rebuild():
+    from cubam_amd import BamHeader
+
+    text = "@HD\tVN:1.6\tSO:coordinate\n@SQ\tSN:chr1\tLN:1000000\n@PG\tID:a\n@PG\tID:b\n"
+    raw = bf.header_bytes(text=text)
+    h = BamHeader.from_bytes(raw + b"trailing record bytes")
+    assert (h.names, h.lengths, h.size) == ([n for n, _ in bf.REFS], [ln for _, ln in bf.REFS], len(raw))
+    assert h.sort_order == "coordinate" and h.lines("@PG") == ["@PG\tID:a", "@PG\tID:b"]
+    assert h.to_bytes() == raw
+    h.require_coordinate_sorted()
+    other = BamHeader.from_bytes(bf.header_bytes(text="@HD\tVN:1.6\tSO:queryname\n"))
+    assert other.sort_order == "queryname"
+    with pytest.raises(ValueError, match="not coordinate-sorted"):
+        other.require_coordinate_sorted("x.bam")
+    assert BamHeader.from_bytes(bf.header_bytes(text="@SQ\tSN:a\tLN:1\n")).sort_order == "unknown"
+    # text is cut at its first NUL; size still counts l_text bytes
+    padded = bf.header_bytes(text="@HD\tSO:unsorted\n\0\0\0\0")
+    hp = BamHeader.from_bytes(padded)
+    assert hp.text == "@HD\tSO:unsorted\n" and hp.size == len(padded)
+    with pytest.raises(ValueError, match="truncated BAM header"):
+        BamHeader.from_bytes(raw[:-3])
+    with pytest.raises(ValueError, match="not a BAM file"):
+        BamHeader.from_bytes(b"BAI\1" + raw[4:])
+
+
+def test_bam_header_read_from_bgzf(tmp_path):
+    from cubam_amd import BamHeader
+
+    p = tmp_path / "h.bam"
+    p.write_bytes(bf.bgzf(bf.bam_bytes(bf.random_reads(20)), 100))
+    h = BamHeader.read(p)
+    assert h.size == len(bf.header_bytes()) and h.names == [n for n, _ in bf.REFS]
