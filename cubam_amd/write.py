@@ -64,7 +64,8 @@ def deflate_blocks(data: torch.Tensor, in_off: torch.Tensor, in_len: torch.Tenso
     bsize = torch.empty(n, dtype=_I32, device=dev)
     crc = torch.empty(n, dtype=torch.uint32, device=dev)
     dws = ws.dev("deflate_ws", L.cbm_deflate_workspace(n))
-    check(L.cbm_deflate(ptr(data), ptr(in_off), ptr(in_len.to(_I32)), n, ptr(slots), ptr(bsize), ptr(crc), ptr(dws),
+    in_len32 = in_len.to(_I32)  # held until the call is enqueued
+    check(L.cbm_deflate(ptr(data), ptr(in_off), ptr(in_len32), n, ptr(slots), ptr(bsize), ptr(crc), ptr(dws),
                         dws.numel(), stream_ptr()), "cbm_deflate")
     sizes = bsize.to(_I64)
     ends = torch.cumsum(sizes, 0)

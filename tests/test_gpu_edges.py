@@ -185,7 +185,8 @@ def test_spurious_bgzf_header_takes_the_serial_chain(cbm, oracle, tmp_path):
     assert blocks.n_blocks == 4 and blocks.n_cand > blocks.n_blocks
     off, _, _, _, used, status = oracle.bgzf_scan(blob, True)
     assert status == 0 and used == len(blob)
-    assert blocks.blk_off.cpu().numpy().tolist() == off.tolist()
+    nonempty = blocks.isize.cpu().numpy() > 0  # the oracle's scan leaves out the empty EOF block
+    assert blocks.blk_off.cpu().numpy()[nonempty].tolist() == off.tolist()
     p = tmp_path / "s.gz"
     p.write_bytes(blob)
     got = b"".join(c.cpu().numpy().tobytes() for c in inflate_chunks(p, chunk_bytes=1 << 20))
