@@ -74,6 +74,7 @@ SYMBOLS = (
     "cbm_bgzf_scan",
     "cbm_inflate_workspace",
     "cbm_inflate",
+    "cbm_inflate_noalloc",
     "cbm_crc32_spans",
     "cbm_record_chain_workspace",
     "cbm_record_chain",
@@ -122,6 +123,8 @@ def load():
     L.cbm_inflate_workspace.argtypes = [i64]
     L.cbm_inflate.restype = ctypes.c_int
     L.cbm_inflate.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(InflateTune), vp, i64, vp]
+    L.cbm_inflate_noalloc.restype = ctypes.c_int
+    L.cbm_inflate_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, ctypes.POINTER(InflateTune), vp]
     L.cbm_crc32_spans.restype = ctypes.c_int
     L.cbm_crc32_spans.argtypes = [vp, vp, vp, i64, vp, vp]
     L.cbm_record_chain_workspace.restype = i64

@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "bgzf_scan.h"
@@ -183,8 +184,8 @@ static int64_t inflate_max_groups() { return (int64_t)device_cus() * 2; }
 template <int NT, int SEG>
 static int launch_wave(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                        const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
-                       uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t* counter,
-                       hipStream_t st) {
+                       uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t max_groups,
+                       int64_t* counter, hipStream_t st) {
     const size_t lds = sizeof(WaveSmem<NT, SEG>);
     static bool attr = false;
     if (!attr) {
@@ -195,7 +196,7 @@ static int launch_wave(const uint8_t* src, const int64_t* blk_off, const int32_t
     // workgroups that fit a CU's 160 KiB of LDS (allocated in 512-byte granules)
     const size_t per_cu = (160 * 1024) / ((lds + 511) / 512 * 512);
     int64_t g = groups > 0 ? groups : (int64_t)device_cus() * (int64_t)(per_cu ? per_cu : 1);
-    if (g > inflate_max_groups()) g = inflate_max_groups();  // the workspace holds that many match lists
+    if (g > max_groups) g = max_groups;  // the workspace holds that many match lists
     if (g > n_blocks) g = n_blocks;
     if (g < 1) g = 1;
     uint64_t* mlists = (uint64_t*)((uint8_t*)counter + INFLATE_WS_HDR);
@@ -290,12 +291,14 @@ int64_t cbm_inflate_workspace(int64_t n_blocks) {
     return INFLATE_WS_HDR + inflate_max_groups() * (int64_t)MLIST_CAP * 8;
 }
 
-int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
-                const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst, uint32_t* crc,
-                int32_t* status, int64_t* result, const cbm_inflate_tune* tune, void* ws, int64_t ws_bytes,
-                void* stream) {
-    hipStream_t st = (hipStream_t)stream;
-    ARG(n_blocks >= 0 && result, "cbm_inflate: bad arguments");
+}  // extern "C"
+
+// cbm_inflate's work once the workspace is settled: ws holds INFLATE_WS_HDR bytes and max_groups
+// match lists (unused by the lane variant)
+static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                        const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                        uint32_t* crc, int32_t* status, int64_t* result, const cbm_inflate_tune* tune, void* ws,
+                        int64_t max_groups, hipStream_t st) {
     int rc = fill(result, 1, 0, st);
     if (rc || n_blocks == 0) return rc;
     ARG(src && blk_off && bsize && isize && crc_expect && out_off && dst && crc && status,
@@ -303,14 +306,13 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
     const int variant = tune ? tune->variant : 0;
     const int64_t groups = tune ? tune->groups : 0;
     if (variant != CBM_INFLATE_LANES) {
-        ARG(ws && ws_bytes >= cbm_inflate_workspace(n_blocks), "cbm_inflate: workspace too small");
         int64_t* counter = (int64_t*)ws;  // [0] next block, [8..] decoder statistics
         if ((rc = fill(counter, 8 + STAT_N, 0, st))) return rc;
         switch (variant) {
 #define WAVE_CASE(V, NT_, SEG_)                                                                               \
     case V:                                                                                                   \
         return launch_wave<NT_, SEG_>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, \
-                                      result, groups, counter, st);
+                                      result, groups, max_groups, counter, st);
             WAVE_CASE(CBM_INFLATE_WG512_S128, 512, 128)
             WAVE_CASE(CBM_INFLATE_WG256, 256, 256)
             WAVE_CASE(CBM_INFLATE_WG128, 128, 256)
@@ -332,8 +334,8 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
     ARG(lds <= 160 * 1024, "cbm_inflate: blocks_per_wave * waves_per_group needs more LDS than a CU has");
     int64_t per_group = (int64_t)bpw * wpg;
     int64_t g = groups > 0 ? groups : (n_blocks + per_group - 1) / per_group;
-    int64_t max_groups = (int64_t)device_cus() * (int64_t)((160 * 1024) / lds);
-    if (g > max_groups) g = max_groups;
+    const int64_t lane_groups = (int64_t)device_cus() * (int64_t)((160 * 1024) / lds);
+    if (g > lane_groups) g = lane_groups;
     if (g < 1) g = 1;
     static bool attr_set = false;
     if (!attr_set) {
@@ -348,6 +350,68 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
     hipLaunchKernelGGL(crc_blocks, dim3((unsigned)cg), dim3(CRC_NT), 0, st, dst, out_off, isize, crc_expect,
                        n_blocks, crc, status, result);
     return check_launch("crc_blocks");
+}
+
+// ------------------------------------------------------------------ inflate without allocation
+// The workspace of cbm_inflate_noalloc: the block counter and statistics header, then one match
+// list for each of at most STATIC_INFLATE_GROUPS workgroups (89.7 MB). Each GPU holds its own copy
+// for the life of the process. Its address in host code comes from hipGetSymbolAddress, once per
+// device; the array's name itself is no device address on the host.
+constexpr int64_t STATIC_INFLATE_GROUPS = 512;
+constexpr int64_t STATIC_INFLATE_BYTES = INFLATE_WS_HDR + STATIC_INFLATE_GROUPS * (int64_t)MLIST_CAP * 8;
+static __device__ __align__(256) uint8_t scratch[STATIC_INFLATE_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MAX_DEVICES];
+
+// the static workspace of the current device (looked up on the first call on each device)
+static int static_scratch(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        set_error("cbm_inflate_noalloc: no current device below %d", MAX_DEVICES);
+        return CBM_ERR_LAUNCH;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("cbm_inflate_noalloc: hipGetSymbolAddress of its scratch: %s", hipGetErrorString(e));
+            return CBM_ERR_LAUNCH;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CBM_OK;
+}
+
+extern "C" {
+
+int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst, uint32_t* crc,
+                int32_t* status, int64_t* result, const cbm_inflate_tune* tune, void* ws, int64_t ws_bytes,
+                void* stream) {
+    ARG(n_blocks >= 0 && result, "cbm_inflate: bad arguments");
+    const bool lanes = tune && tune->variant == CBM_INFLATE_LANES;
+    ARG(lanes || n_blocks == 0 || (ws && ws_bytes >= cbm_inflate_workspace(n_blocks)),
+        "cbm_inflate: workspace too small");
+    return inflate_impl(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, result, tune, ws,
+                        inflate_max_groups(), (hipStream_t)stream);
+}
+
+int cbm_inflate_noalloc(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                        const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                        uint32_t* crc, int32_t* status, int64_t* result, const cbm_inflate_tune* tune, void* stream) {
+    ARG(n_blocks >= 0 && result, "cbm_inflate_noalloc: bad arguments");
+    ARG(!tune || tune->groups <= STATIC_INFLATE_GROUPS,
+        "cbm_inflate_noalloc: at most 512 workgroups (its static scratch holds 512 match lists)");
+    void* ws = nullptr;
+    const int rc = static_scratch(&ws);
+    if (rc) return rc;
+    const int64_t cap = inflate_max_groups() < STATIC_INFLATE_GROUPS ? inflate_max_groups() : STATIC_INFLATE_GROUPS;
+    return inflate_impl(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, result, tune, ws,
+                        cap, (hipStream_t)stream);
 }
 
 int cbm_crc32_spans(const uint8_t* data, const int64_t* off, const int64_t* len, int64_t n, uint32_t* out,

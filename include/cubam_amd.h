@@ -126,6 +126,25 @@ int cbm_inflate(const uint8_t* src_d, const int64_t* blk_off_d, const int32_t* b
                 uint32_t* crc_d, int32_t* status_d, int64_t* result_d, const cbm_inflate_tune* tune, void* ws_d,
                 int64_t ws_bytes, void* stream);
 
+/*
+ * cbm_inflate without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. It calls no
+ * allocating routine: the block counter, statistics and match lists live in a
+ * static device array of the library, 89.7 MB on each GPU it runs on, held
+ * until the process ends, found once per device with hipGetSymbolAddress (the
+ * calling thread's current device must be the stream's). It holds 512 match
+ * lists, so tune->groups above 512 is rejected (CBM_ERR_ARG); the default
+ * launch uses min(512, two a CU). Results are those of cbm_inflate with the
+ * same tune. Every call on a GPU shares that GPU's array, and each call clears
+ * what must start at zero: calls that overlap on one GPU (two streams, two
+ * threads) race and must be serialised by the caller.
+ */
+int cbm_inflate_noalloc(const uint8_t* src_d, const int64_t* blk_off_d, const int32_t* bsize_d,
+                        const int32_t* isize_d, const uint32_t* crc_expect_d, const int64_t* out_off_d,
+                        int64_t n_blocks, uint8_t* dst_d, uint32_t* crc_d, int32_t* status_d, int64_t* result_d,
+                        const cbm_inflate_tune* tune, void* stream);
+
 /* CRC32 (zlib) of n spans data_d[off[i] : off[i] + len[i]], one wavefront per span. Replaces
    crc32_spans (src/cubam/_kernels.py:106-141). */
 int cbm_crc32_spans(const uint8_t* data_d, const int64_t* off_d, const int64_t* len_d, int64_t n, uint32_t* out_d,
