@@ -181,29 +181,82 @@ static int repair_store(const RecWs& w, std::vector<int64_t>& f, const std::vect
 // workgroups of the persistent inflate launch: the LDS of a CU holds two block buffers
 static int64_t inflate_max_groups() { return (int64_t)device_cus() * 2; }
 
+// one launch of inflate_wave<NT, SEG, TOK>: a persistent grid taking blocks from next_block
+template <int NT, int SEG, bool TOK>
+static int launch_inflate_wave(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                               const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                               uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t max_groups,
+                               int64_t* next_block, uint64_t* mlists, int64_t* stats, const int64_t* idx,
+                               const int64_t* n_dev, uint64_t* pool, int32_t* mcount, hipStream_t st) {
+    const size_t lds = sizeof(WaveSmem<NT, SEG, TOK>);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)inflate_wave<NT, SEG, TOK>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    // workgroups that fit a CU's 160 KiB of LDS (allocated in 512-byte granules); the decode phase
+    // asks for four wavefronts a SIMD, so at most 16 of its one-wavefront workgroups run on a CU
+    size_t per_cu = (160 * 1024) / ((lds + 511) / 512 * 512);
+    if (TOK && per_cu > 16) per_cu = 16;
+    int64_t g = groups > 0 ? groups : (int64_t)device_cus() * (int64_t)(per_cu ? per_cu : 1);
+    if (g > max_groups) g = max_groups;  // the workspace holds that many match lists (one-phase)
+    if (g > n_blocks) g = n_blocks;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL((inflate_wave<NT, SEG, TOK>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
+                       crc_expect, out_off, n_blocks, dst, crc, status, result, (unsigned long long*)next_block,
+                       mlists, (unsigned long long*)stats, idx, n_dev, pool, mcount);
+    return check_launch("inflate_wave");
+}
+
 template <int NT, int SEG>
 static int launch_wave(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                        const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
                        uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t max_groups,
                        int64_t* counter, hipStream_t st) {
-    const size_t lds = sizeof(WaveSmem<NT, SEG>);
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)inflate_wave<NT, SEG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
-        attr = true;
-    }
-    // workgroups that fit a CU's 160 KiB of LDS (allocated in 512-byte granules)
-    const size_t per_cu = (160 * 1024) / ((lds + 511) / 512 * 512);
-    int64_t g = groups > 0 ? groups : (int64_t)device_cus() * (int64_t)(per_cu ? per_cu : 1);
-    if (g > max_groups) g = max_groups;  // the workspace holds that many match lists
-    if (g > n_blocks) g = n_blocks;
-    if (g < 1) g = 1;
     uint64_t* mlists = (uint64_t*)((uint8_t*)counter + INFLATE_WS_HDR);
-    hipLaunchKernelGGL((inflate_wave<NT, SEG>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
-                       crc_expect, out_off, n_blocks, dst, crc, status, result, (unsigned long long*)counter, mlists,
-                       (unsigned long long*)counter + 8);
-    return check_launch("inflate_wave");
+    return launch_inflate_wave<NT, SEG, false>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                               status, result, groups, max_groups, counter, mlists, counter + 8,
+                                               nullptr, nullptr, nullptr, nullptr, st);
+}
+
+// bytes the two-phase inflate adds to the workspace for n blocks: the deferred-block list, the
+// match counts and a match list of MCAP_TOK entries a block
+static int64_t tok_ws_bytes(int64_t n) { return ((12 * n + 255) / 256) * 256 + n * (int64_t)MCAP_TOK * 8; }
+
+// The two-phase inflate: the decode phase (a wavefront a block, SEG-bit segments, literals to
+// dst, matches to the block's list), the resolve phase (the matches, in dst), the CRC32 of every
+// block the decode phase finished, then the one-phase kernel over the blocks it left.
+// ws: the one-phase workspace (counter, statistics, max_groups match lists), then tok_ws_bytes(n).
+template <int SEG>
+static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                      const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                      uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, void* ws, int64_t ws_groups,
+                      int64_t max_groups, hipStream_t st) {
+    int64_t* counter = (int64_t*)ws;  // [0] decode-phase next block, [1] one-phase next block, [2] blocks left
+    uint64_t* mlists = (uint64_t*)((uint8_t*)ws + INFLATE_WS_HDR);
+    uint8_t* tok = (uint8_t*)mlists + ws_groups * (int64_t)MLIST_CAP * 8;
+    int64_t* idx = (int64_t*)tok;
+    int32_t* mcount = (int32_t*)(idx + n_blocks);
+    uint64_t* pool = (uint64_t*)(tok + ((12 * n_blocks + 255) / 256) * 256);
+    int rc = launch_inflate_wave<64, SEG, true>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                status, result, groups, (int64_t)1 << 40, counter, nullptr,
+                                                counter + 8, nullptr, nullptr, pool, mcount, st);
+    if (rc) return rc;
+    // the resolve workgroups that fit a CU's LDS
+    const int64_t res_per_cu = (160 * 1024) / (int64_t)((sizeof(ResSmem) + 511) / 512 * 512);
+    int64_t g = (n_blocks + RES_WAVES - 1) / RES_WAVES;
+    if (g > (int64_t)device_cus() * res_per_cu) g = (int64_t)device_cus() * res_per_cu;
+    hipLaunchKernelGGL(inflate_resolve, dim3((unsigned)g), dim3(RES_NT), 0, st, dst, out_off, isize, crc_expect, crc,
+                       status, result, pool, mcount, n_blocks);
+    if ((rc = check_launch("inflate_resolve"))) return rc;
+    int64_t lg = (n_blocks + 255) / 256;
+    if (lg > 1024) lg = 1024;
+    hipLaunchKernelGGL(inflate_defer_list, dim3((unsigned)lg), dim3(256), 0, st, status, n_blocks, idx, counter + 2);
+    if ((rc = check_launch("inflate_defer_list"))) return rc;
+    return launch_inflate_wave<512, 128, false>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                status, result, 0, max_groups, counter + 1, mlists, counter + 8, idx,
+                                                counter + 2, nullptr, nullptr, st);
 }
 
 }  // namespace cbm
@@ -287,23 +340,27 @@ int cbm_bgzf_scan(const uint8_t* buf, int64_t n, int eof, int64_t cap, int64_t* 
 
 // ------------------------------------------------------------------ kernel 2
 int64_t cbm_inflate_workspace(int64_t n_blocks) {
-    (void)n_blocks;
-    return INFLATE_WS_HDR + inflate_max_groups() * (int64_t)MLIST_CAP * 8;
+    if (n_blocks < 0) return -1;
+    return INFLATE_WS_HDR + inflate_max_groups() * (int64_t)MLIST_CAP * 8 + tok_ws_bytes(n_blocks);
 }
 
 }  // extern "C"
 
+static bool tok_variant(int v) { return v >= CBM_INFLATE_TOK64_S512 && v <= CBM_INFLATE_TOK64_S256; }
+
 // cbm_inflate's work once the workspace is settled: ws holds INFLATE_WS_HDR bytes and max_groups
-// match lists (unused by the lane variant)
+// match lists (unused by the lane variant), and with `tok` the two-phase inflate's tok_ws_bytes
+// after inflate_max_groups() lists; without it a two-phase variant runs the one-phase default
 static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                         const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
                         uint32_t* crc, int32_t* status, int64_t* result, const cbm_inflate_tune* tune, void* ws,
-                        int64_t max_groups, hipStream_t st) {
+                        int64_t max_groups, bool tok, hipStream_t st) {
     int rc = fill(result, 1, 0, st);
     if (rc || n_blocks == 0) return rc;
     ARG(src && blk_off && bsize && isize && crc_expect && out_off && dst && crc && status,
         "cbm_inflate: null pointer");
-    const int variant = tune ? tune->variant : 0;
+    int variant = tune ? tune->variant : CBM_INFLATE_DEFAULT;
+    if (tok_variant(variant) && !tok) variant = CBM_INFLATE_WG512_S128;
     const int64_t groups = tune ? tune->groups : 0;
     if (variant != CBM_INFLATE_LANES) {
         int64_t* counter = (int64_t*)ws;  // [0] next block, [8..] decoder statistics
@@ -324,6 +381,14 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
             WAVE_CASE(CBM_INFLATE_WG64_S256, 64, 256)
             WAVE_CASE(CBM_INFLATE_WG384_S128, 384, 128)
 #undef WAVE_CASE
+#define TOK_CASE(V, SEG_)                                                                                    \
+    case V:                                                                                                  \
+        return launch_tok<SEG_>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, \
+                                result, groups, ws, inflate_max_groups(), max_groups, st);
+            TOK_CASE(CBM_INFLATE_TOK64_S512, 512)
+            TOK_CASE(CBM_INFLATE_TOK64_S1024, 1024)
+            TOK_CASE(CBM_INFLATE_TOK64_S256, 256)
+#undef TOK_CASE
             default: ARG(false, "cbm_inflate: unknown variant");
         }
     }
@@ -397,7 +462,7 @@ int cbm_inflate(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize
     ARG(lanes || n_blocks == 0 || (ws && ws_bytes >= cbm_inflate_workspace(n_blocks)),
         "cbm_inflate: workspace too small");
     return inflate_impl(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, result, tune, ws,
-                        inflate_max_groups(), (hipStream_t)stream);
+                        inflate_max_groups(), true, (hipStream_t)stream);
 }
 
 int cbm_inflate_noalloc(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
@@ -411,7 +476,7 @@ int cbm_inflate_noalloc(const uint8_t* src, const int64_t* blk_off, const int32_
     if (rc) return rc;
     const int64_t cap = inflate_max_groups() < STATIC_INFLATE_GROUPS ? inflate_max_groups() : STATIC_INFLATE_GROUPS;
     return inflate_impl(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, result, tune, ws,
-                        cap, (hipStream_t)stream);
+                        cap, false, (hipStream_t)stream);
 }
 
 int cbm_crc32_spans(const uint8_t* data, const int64_t* off, const int64_t* len, int64_t n, uint32_t* out,

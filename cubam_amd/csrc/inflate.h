@@ -54,6 +54,10 @@ struct FixedTables {
 
 enum { TAB_CODES = 0, TAB_LENS = 1, TAB_DISTS = 2 };
 
+// status of a block the decode phase of the two-phase inflate leaves to the one-phase kernel
+// (never left in a status array the caller sees)
+constexpr int BLOCK_DEFER = 0xff;
+
 __device__ __forceinline__ uint32_t bitrev(uint32_t c, int len) { return __builtin_bitreverse32(c) >> (32 - len); }
 
 // Canonical Huffman decode table for code lengths lens[0:nsym) (RFC 1951 section 3.2.2).
@@ -453,6 +457,7 @@ __global__ void __launch_bounds__(CRC_NT) crc_blocks(const uint8_t* dst, const i
     int wave = threadIdx.x / CBM_WAVE, lane = threadIdx.x % CBM_WAVE;
     int64_t nw = (int64_t)gridDim.x * (CRC_NT / CBM_WAVE);
     for (int64_t b = (int64_t)blockIdx.x * (CRC_NT / CBM_WAVE) + wave; b < n; b += nw) {
+        if (status[b] == BLOCK_DEFER) continue;  // decoded again, CRC32 included, by the one-phase kernel
         int64_t len = isize[b];
         int64_t e = len - (int64_t)(63 - lane) * CRC_SEG, s = e - CRC_SEG;
         if (s < 0) s = 0;

@@ -38,13 +38,14 @@
 namespace cbm {
 
 enum { SV_POS = 0, SV_FINAL, SV_TYPE, SV_ERR, SV_LEN, SV_SRC, SV_OP, SV_CLEAR, SV_TEND, SV_CHANGED, SV_DONE, SV_OK,
-       SV_BLOCK, SV_MAX, SV_SITE, SV_F0, SV_F1, SV_F2, SV_N };
+       SV_BLOCK, SV_MAX, SV_SITE, SV_F0, SV_F1, SV_F2, SV_MCNT, SV_N };
 
 // matches each thread holds in pass D (fewer in the widest workgroups, to save registers)
 template <int NT>
 constexpr int md_k() { return NT >= 512 ? 4 : 8; }
 
-template <int NT, int SEG>
+// TOK: the decode phase of the two-phase inflate (tokens to global memory, no output window in LDS)
+template <int NT, int SEG, bool TOK = false>
 struct WaveSmem {
     // staged compressed bits of a round; at least 1 << LBITS words, the table build's scratch
     static constexpr int IN_WORDS = NT * SEG / 32 + 8 > (1 << LBITS) + 8 ? NT * SEG / 32 + 8 : (1 << LBITS) + 8;
@@ -70,7 +71,7 @@ struct WaveSmem {
     int32_t lnext[16];
     int32_t chunkcnt[5][16];
     int32_t sv[SV_N];  // shared scalars (SV_*)
-    alignas(16) uint8_t out[65536 + 16];
+    alignas(16) uint8_t out[TOK ? 16 : 65536 + 16];  // the block's output window (not with TOK)
 };
 
 
@@ -576,19 +577,32 @@ __constant__ CrcTables kCrcTab = make_crc_tables();
 // ------------------------------------------------------------------ the kernel
 // two workgroups share a CU (the LDS holds two output windows): ask for the registers that let
 // both run, NT / 128 wavefronts per SIMD
-template <int NT, int SEG>
-__global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(const uint8_t* src, const int64_t* blk_off,
+//
+// TOK (the decode phase of the two-phase inflate): no output window. Literals go straight to their
+// place in dst and matches to the block's list in `pool` (MCAP_TOK entries a block, in output
+// order; mcount[b] of them); status[b] is CBM_BLOCK_OK, or BLOCK_DEFER for a block this phase
+// does not finish (any check that fails, or more matches than its list holds), which the
+// one-phase kernel decodes again. No CRC32 here. Speculative walks start at most TOK_OVL bits
+// before their segment.
+// idx / n_dev (may be null): decode blocks idx[0 .. *n_dev) instead of 0 .. n_blocks.
+constexpr int MCAP_TOK = 6144;
+#ifndef CBM_TOK_OVL
+#define CBM_TOK_OVL 256
+#endif
+template <int NT, int SEG, bool TOK = false>
+__global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) inflate_wave(const uint8_t* src, const int64_t* blk_off,
                                                    const int32_t* bsize, const int32_t* isize,
                                                    const uint32_t* expect, const int64_t* out_off,
                                                    int64_t n_blocks, uint8_t* dst, uint32_t* crc_out,
                                                    int32_t* status, int64_t* bad_count,
                                                    unsigned long long* next_block, uint64_t* mlists,
-                                                   unsigned long long* stats) {
-    using S = WaveSmem<NT, SEG>;
+                                                   unsigned long long* stats, const int64_t* idx,
+                                                   const int64_t* n_dev, uint64_t* pool, int32_t* mcount) {
+    using S = WaveSmem<NT, SEG, TOK>;
     extern __shared__ __align__(16) uint8_t smem_raw[];
     S& s = *(S*)smem_raw;
     const int tid = threadIdx.x;
-    uint64_t* mlist = mlists + (size_t)blockIdx.x * MLIST_CAP;
+    uint64_t* mlist = TOK ? nullptr : mlists + (size_t)blockIdx.x * MLIST_CAP;
 #if CBM_INFLATE_STATS
     // cycles per phase, kept by thread 0 and added to stats[] once per block
     long long acc[STAT_N] = {};
@@ -633,8 +647,10 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
     for (;;) {
         if (tid == 0) s.sv[SV_BLOCK] = (int)atomicAdd(next_block, 1ull);
         __syncthreads();
-        const int64_t b = s.sv[SV_BLOCK];
-        if (b >= n_blocks) break;
+        const int64_t kb = s.sv[SV_BLOCK];
+        if (kb >= (n_dev ? *n_dev : n_blocks)) break;
+        const int64_t b = idx ? idx[kb] : kb;
+        uint64_t* blist = TOK ? pool + (size_t)b * MCAP_TOK : nullptr;  // this block's match list (TOK)
         const uint8_t* in = src + blk_off[b] + 18;
         const int in_bytes = bsize[b] - 26;
         const int in_bits = 8 * in_bytes;
@@ -646,6 +662,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             s.sv[SV_ERR] = CBM_BLOCK_OK;
             s.sv[SV_SITE] = 0;
             s.sv[SV_FINAL] = 0;
+            s.sv[SV_MCNT] = 0;
         }
         __syncthreads();
         while (!s.sv[SV_FINAL] && s.sv[SV_ERR] == CBM_BLOCK_OK) {
@@ -699,7 +716,12 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
             const int type = s.sv[SV_TYPE];
             if (type == 0) {
                 const int len = s.sv[SV_LEN], sp = s.sv[SV_SRC], op = s.sv[SV_OP];
-                for (int i = tid; i < len; i += NT) s.out[op + i] = in[sp + i];
+                if constexpr (TOK) {
+                    uint8_t* d = dst + out_off[b] + op;
+                    for (int i = tid; i < len; i += NT) d[i] = in[sp + i];
+                } else {
+                    for (int i = tid; i < len; i += NT) s.out[op + i] = in[sp + i];
+                }
                 __syncthreads();
                 if (tid == 0) s.sv[SV_OP] = op + len;
                 __syncthreads();
@@ -918,7 +940,8 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // checkpoint 0 is its first symbol start at or after a, and checkpoint j the start
                 // j * CP_STEPS symbols later, each with the output bytes and matches counted from
                 // checkpoint 0 up to there; and the count to its exit.
-                const int OVL = seg * CBM_INFLATE_OVL_HALVES / 2;
+                const int OVL = TOK ? min(seg * CBM_INFLATE_OVL_HALVES / 2, CBM_TOK_OVL)
+                                    : seg * CBM_INFLATE_OVL_HALVES / 2;
 #ifdef CBM_INFLATE_NCP
                 constexpr int NCP = CBM_INFLATE_NCP, CP_STEPS = CBM_INFLATE_CP_STEPS;
 #else
@@ -1045,7 +1068,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 const int total = (int)(uint32_t)tot2, mtotal = (int)(tot2 >> 32);
                 const int op = s.sv[SV_OP];
                 const int tstop = tend < NT ? s.stop_[tend] : STOP_NONE;
-                if (tstop == STOP_BAD || op + total > cap) {
+                if (tstop == STOP_BAD || op + total > cap || (TOK && s.sv[SV_MCNT] + mtotal > MCAP_TOK)) {
                     __syncthreads();
                     if (tid == 0) {
                         s.sv[SV_ERR] = tstop == STOP_BAD ? CBM_BLOCK_DEFLATE : CBM_BLOCK_SIZE;
@@ -1058,7 +1081,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // pass C: literals go to their place; matches go to the list, in output order
                 if (live && cnt) {
                     const int e = s.exit_[tid];
-                    int o = op + ob, k = mbase;
+                    int o = op + ob, k = mbase + (TOK ? s.sv[SV_MCNT] : 0);
                     bool bad = false;
                     Win w;
                     win_at(w, s, tid == 0 ? base : s.exit_[tid - 1], wb);
@@ -1068,10 +1091,15 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                             bad = y.kind == 1;  // a distance reaching before the block's first byte
                             break;
                         }
-                        if (y.kind == 0) s.out[o] = (uint8_t)y.val;
-                        else
-                            mlist[k++] = (uint64_t)(uint32_t)o | ((uint64_t)(uint32_t)y.dist << 17) |
-                                         ((uint64_t)(uint32_t)y.val << 33);
+                        const uint64_t mrec = (uint64_t)(uint32_t)o | ((uint64_t)(uint32_t)y.dist << 17) |
+                                              ((uint64_t)(uint32_t)y.val << 33);
+                        if constexpr (TOK) {
+                            if (y.kind == 0) dst[out_off[b] + o] = (uint8_t)y.val;
+                            else blist[k++] = mrec;
+                        } else {
+                            if (y.kind == 0) s.out[o] = (uint8_t)y.val;
+                            else mlist[k++] = mrec;
+                        }
                         o += y.kind == 0 ? 1 : y.val;
                     }
                     if (bad) {
@@ -1092,6 +1120,7 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                 // least. Other variants use the frontier F, the destination of the first unapplied
                 // match: every byte before F is final. Rounds with more matches than the threads
                 // hold go through one wavefront in list order.
+                if constexpr (!TOK) {  // TOK: the resolve phase applies the block's matches
                 constexpr int MD_K = md_k<NT>();
 #if CBM_INFLATE_STATS
                 if (tid == 0) acc[STAT_MSUM] += mtotal, acc[STAT_MBIG] += mtotal > MD_K * NT;
@@ -1274,11 +1303,13 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
                         }
                     }
                 }
+                }  // !TOK
                 __syncthreads();
                 PHASE(STAT_PASS_D);
                 COUNT(STAT_ROUNDS);
                 if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
                 if (tid == 0) {
+                    s.sv[SV_MCNT] += mtotal;
                     s.sv[SV_OP] = op + total;
                     s.sv[SV_POS] = tend < NT ? s.exit_[tend] : s.exit_[NT - 1];
                 }
@@ -1293,6 +1324,16 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
         const int op = s.sv[SV_OP];
         if (err == CBM_BLOCK_OK && s.sv[SV_POS] > in_bits) err = CBM_BLOCK_DEFLATE, site = SITE_OVERRUN;
         if (err == CBM_BLOCK_OK && op != cap) err = CBM_BLOCK_SIZE, site = SITE_SIZE;
+        if constexpr (TOK) {
+            // matches and CRC32 come later (resolve and CRC kernels); a block with any failed check
+            // is decoded again by the one-phase kernel, which reports it exactly
+            if (tid == 0) {
+                status[b] = err == CBM_BLOCK_OK ? CBM_BLOCK_OK : BLOCK_DEFER;
+                mcount[b] = s.sv[SV_MCNT];
+            }
+            __syncthreads();
+            continue;
+        }
         const int nout = err == CBM_BLOCK_OK ? cap : (op < cap ? op : cap);
         uint8_t* d = dst + out_off[b];
         {
@@ -1359,6 +1400,183 @@ __global__ void __launch_bounds__(NT, (NT >= 256 ? NT / 128 : 1)) inflate_wave(c
     }
 #undef PHASE
 #undef COUNT
+}
+
+// ------------------------------------------------------------------ resolve phase (two-phase inflate)
+// A wavefront per block that the decode phase finished (status OK), then the block's CRC32.
+// The block's matches (in output order) go in windows of up to 64, a match a lane, whose output
+// span [lo, hi) fits RES_SPAN bytes: the span is staged in LDS from dst (its literals are in
+// place), the bytes the window's matches write are marked pending in a bitmap, and each step
+// applies every match whose source holds no pending byte (bytes before lo are final in dst:
+// earlier windows were stored); then the span goes back to dst. The lowest pending match always
+// has a final source, so every step applies one at least. A match reads only final bytes,
+// [src, src + len) or the dist bytes before dest that an overlapping copy repeats.
+constexpr int RES_NT = 256;
+constexpr int RES_WAVES = RES_NT / CBM_WAVE;
+constexpr int RES_SPAN = 4096;
+
+struct ResSmem {
+    uint32_t crc_t[4][256];
+    uint32_t crc_x[64];
+    alignas(16) uint8_t span[RES_WAVES][RES_SPAN + 16];
+    uint32_t pend[RES_WAVES][RES_SPAN / 32 + 2];
+};
+
+__device__ __forceinline__ void res_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// zlib CRC32 (pre- and post-inverted) of p[0:n) in global memory: bytes up to a 16-byte boundary,
+// then 16 bytes a load (loads of a 64-byte stretch issued together), slicing-by-4 from LDS tables
+__device__ __forceinline__ uint32_t crc_bytes16(const uint8_t* p, int n, const uint32_t (*T)[256]) {
+    uint32_t c = 0xFFFFFFFFu;
+    int k = 0;
+    while (k < n && (((uintptr_t)(p + k)) & 15)) c = T[0][(c ^ p[k++]) & 255] ^ (c >> 8);
+    for (; k + 64 <= n; k += 64) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = *(const uint4*)(p + k + 16 * u);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t w[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                c ^= w[v];
+                c = T[3][c & 255] ^ T[2][(c >> 8) & 255] ^ T[1][(c >> 16) & 255] ^ T[0][c >> 24];
+            }
+        }
+    }
+    for (; k + 4 <= n; k += 4) {
+        c ^= *(const uint32_t*)(p + k);
+        c = T[3][c & 255] ^ T[2][(c >> 8) & 255] ^ T[1][(c >> 16) & 255] ^ T[0][c >> 24];
+    }
+    for (; k < n; k++) c = T[0][(c ^ p[k]) & 255] ^ (c >> 8);
+    return c ^ 0xFFFFFFFFu;
+}
+
+// len bytes from g (dst) to d (LDS), 32 a pass: the aligned words that hold the pass's bytes are
+// loaded together (at most 3 bytes outside the range, inside the same buffer), then realigned
+// with v_alignbyte and written a byte at a time
+__device__ __forceinline__ void res_copy_far(const uint8_t* g, uint8_t* d, int len) {
+    for (int i0 = 0; i0 < len; i0 += 32) {
+        const int sh = (int)((uintptr_t)(g + i0) & 3);
+        const uint32_t* w = (const uint32_t*)(g + i0 - sh);
+        const int nb = len - i0 < 32 ? len - i0 : 32;
+        const int nw = (nb + sh + 3) >> 2;
+        uint32_t x[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) x[j] = j < nw ? w[j] : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t v = __builtin_amdgcn_alignbyte(x[j + 1], x[j], (uint32_t)sh);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (4 * j + k < nb) d[i0 + 4 * j + k] = (uint8_t)(v >> (8 * k));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(RES_NT) inflate_resolve(uint8_t* dst, const int64_t* out_off, const int32_t* isize,
+                                                          const uint32_t* expect, uint32_t* crc_out, int32_t* status,
+                                                          int64_t* bad, const uint64_t* pool, const int32_t* mcount,
+                                                          int64_t n_blocks) {
+    __shared__ ResSmem s;
+    crc_tables(s.crc_t, s.crc_x);
+    const int wv = (int)threadIdx.x / CBM_WAVE, lane = (int)threadIdx.x % CBM_WAVE;
+    uint8_t* sp = s.span[wv];
+    uint32_t* pend = s.pend[wv];
+    for (int64_t b = (int64_t)blockIdx.x * RES_WAVES + wv; b < n_blocks; b += (int64_t)gridDim.x * RES_WAVES) {
+        if (status[b] != CBM_BLOCK_OK) continue;  // left to the one-phase kernel
+        const int M = mcount[b];
+        uint8_t* out = dst + out_off[b];
+        const uint64_t* ml = pool + (size_t)b * MCAP_TOK;
+        bool stuck = false;
+        for (int k0 = 0; k0 < M && !stuck;) {
+            const int k = k0 + lane;
+            const uint64_t m = k < M ? ml[k] : 0;
+            const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff), len = (int)((m >> 33) & 0x1ff);
+            const int lo = __builtin_amdgcn_readlane(dest, 0);
+            // dest + len grows with k (outputs are disjoint and in order): the lanes that fit are a prefix
+            const uint64_t fit = __ballot(k < M && dest + len - lo <= RES_SPAN);
+            const int c = __popcll(fit);  // lane 0's match (at most 258 bytes) always fits
+            const bool mine = lane < c;
+            const int hi = __builtin_amdgcn_readlane(dest + len, c - 1);
+            const int nspan = hi - lo;
+            for (int i = lane; i < nspan; i += CBM_WAVE) sp[i] = out[lo + i];
+            for (int w = lane; w <= (nspan >> 5); w += CBM_WAVE) pend[w] = 0;
+            const int src = dest - dist, srcn = dist < len ? dist : len;
+            // far matches (the source wholly before the span, so final in dst, and no overlap) need
+            // nothing of the window: each lane copies its own first, the loads of a pass together
+            const bool far = mine && dist >= len && src + len <= lo;
+            if (far) res_copy_far(out + src, sp + (dest - lo), len);
+            __threadfence_block();
+            // the others wait on the bytes the window writes
+            const bool near = mine && !far;
+            if (near) mark_bits(pend, dest - lo, len, true);
+            __threadfence_block();
+            bool left = near;
+            int idle = 0;
+            while (__ballot(left)) {
+                bool ready = false;
+                if (left) {
+                    const int a = src < lo ? lo : src, e = src + srcn;
+                    ready = !(a < e && any_bits(pend, a - lo, e - lo));
+                }
+                if (!__ballot(ready) && ++idle > (1 << 12)) {  // cannot happen (the lowest is ready):
+                    stuck = true;                               // leave the block to the one-phase kernel
+                    break;
+                }
+                if (ready) {
+                    // eight bytes a pass, loads before stores: every byte read is final (an
+                    // overlapping copy repeats [src, dest), r = i mod dist)
+                    int r = 0;
+                    for (int i0 = 0; i0 < len; i0 += 8) {
+                        uint8_t t[8];
+#pragma unroll
+                        for (int j = 0; j < 8; j++) {
+                            const int i = i0 + j;
+                            const int q = src + (dist >= len ? i : r);
+                            t[j] = i < len ? (q < lo ? out[q] : sp[q - lo]) : 0;
+                            r = r + 1 == dist ? 0 : r + 1;
+                        }
+#pragma unroll
+                        for (int j = 0; j < 8; j++)
+                            if (i0 + j < len) sp[dest - lo + i0 + j] = t[j];
+                    }
+                }
+                __threadfence_block();  // the bytes land before their bits clear
+                if (ready) mark_bits(pend, dest - lo, len, false);
+                __threadfence_block();
+                left = left && !ready;
+            }
+            for (int i = lane; i < nspan; i += CBM_WAVE) out[lo + i] = sp[i];
+            res_fence();  // later windows read these bytes from dst
+            k0 += c;
+        }
+        if (stuck) {
+            if (lane == 0) status[b] = BLOCK_DEFER;
+            continue;
+        }
+        // CRC32: lane j takes the j-th of 64 stretches of 1,024 bytes counted back from the end
+        const int L = isize[b];
+        const int e = L - (CBM_WAVE - 1 - lane) * 1024, st = e - 1024 > 0 ? e - 1024 : 0;
+        uint32_t c = e > st ? crc_mulmod(s.crc_x[CBM_WAVE - 1 - lane], crc_bytes16(out + st, e - st, s.crc_t)) : 0;
+        for (int d = 32; d; d >>= 1) c ^= __shfl_xor(c, d, CBM_WAVE);
+        if (lane == 0) {
+            crc_out[b] = c;
+            if (c != expect[b]) {
+                status[b] = CBM_BLOCK_CRC;
+                atomicAdd((unsigned long long*)bad, 1ull);
+            }
+        }
+    }
+}
+
+// the blocks left to the one-phase kernel, listed (in any order) with their count
+__global__ void inflate_defer_list(const int32_t* status, int64_t n_blocks, int64_t* idx, int64_t* count) {
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_blocks; b += (int64_t)gridDim.x * blockDim.x)
+        if (status[b] == BLOCK_DEFER) idx[atomicAdd((unsigned long long*)count, 1ull)] = b;
 }
 
 }  // namespace cbm

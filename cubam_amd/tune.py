@@ -35,7 +35,7 @@ from cubam_amd import _decode, _lib
 
 #: workgroup variants, then the lane variant at 1, 2, 4 and 8 blocks per workgroup
 CANDIDATES = tuple(
-    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10)]
+    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13)]
     + [(_lib.INFLATE_LANES, bpw, 1, 0) for bpw in (1, 2, 4, 8)]
     + [(_lib.INFLATE_LANES, 4, 2, 0), (_lib.INFLATE_LANES, 8, 2, 0)]
 )

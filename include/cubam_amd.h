@@ -97,7 +97,14 @@ enum {
     CBM_INFLATE_WG64_S256 = 8,  /* NT 64, SEG 256 */
     CBM_INFLATE_WG384_S128 = 9, /* NT 384, SEG 128 */
     CBM_INFLATE_WG256 = 10,     /* NT 256, SEG 256 */
-    CBM_INFLATE_VARIANTS = 11
+    /* two-phase: a wavefront decodes each block in SEG-bit segments, literals straight to dst_d and
+       matches to a list; a resolve pass applies the matches in dst_d; then the CRC32 check. Blocks
+       the first phase cannot finish go through CBM_INFLATE_WG512_S128 (same results) */
+    CBM_INFLATE_TOK64_S512 = 11,
+    CBM_INFLATE_TOK64_S1024 = 12,
+    CBM_INFLATE_TOK64_S256 = 13,
+    CBM_INFLATE_VARIANTS = 14,
+    CBM_INFLATE_DEFAULT = CBM_INFLATE_WG512_S128 /* variant 0 and a null tune */
 };
 
 /* Launch shape of the inflate kernel; zeros select the defaults (or the autotuned choice). */

@@ -72,6 +72,7 @@ def test_hetero_blocks_bytes_and_crc(cbm, oracle):
 @pytest.mark.parametrize("variant,bpw,wpg,groups", [(0, 0, 0, 0), (0, 0, 0, 7), (2, 0, 0, 0), (3, 0, 0, 0),
                                                     (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0), (7, 0, 0, 0),
                                                     (8, 0, 0, 0), (9, 0, 0, 0), (10, 0, 0, 0), (10, 0, 0, 3),
+                                                    (11, 0, 0, 0), (11, 0, 0, 5), (12, 0, 0, 0), (13, 0, 0, 0),
                                                     (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0), (1, 32, 1, 0)])
 def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     from cubam_amd import _lib
@@ -334,8 +335,9 @@ def test_crc32_spans_matches_zlib(cbm):
     assert got == [zlib.crc32(raw[o : o + s].tobytes()) for o, s in zip(off, size)]
 
 
+@pytest.mark.parametrize("variant", [0, 11])
 @pytest.mark.parametrize("where", ["header", "anywhere"])
-def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where):
+def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant):
     """Blocks with a few bits flipped in their dynamic Huffman header (the code-length code and
     the code lengths: "header") or anywhere in the compressed data, with the trailer of the
     intact payload, among intact blocks. Against zlib on
@@ -370,8 +372,8 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where):
         blob += block
     blob += bf.EOF_BLOCK
     arr = np.frombuffer(bytes(blob), np.uint8)
-    _say("corrupt blocks:", where, len(off), "blocks")
-    blocks, inf = _gpu_blocks(arr)
+    _say("corrupt blocks:", where, len(off), "blocks, variant", variant)
+    blocks, inf = _gpu_blocks(arr, _lib.InflateTune(variant, 0, 0, 0))
     assert blocks.n_blocks == len(off) + 1  # and the EOF block
     st = inf.status.cpu().numpy()[: len(off)] & 255
     data = inf.data[: inf.n].cpu().numpy()
@@ -388,3 +390,35 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where):
         if ost[k] == ok:
             assert data[o : o + isz[k]].tobytes() == payloads[k]
         o += isz[k]
+
+
+@pytest.mark.parametrize("variant", [11, 12, 13])
+def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant):
+    """Blocks with more matches than the decode phase's list holds (3-byte matches all through),
+    stored blocks, fixed-Huffman blocks, and ordinary ones, mixed: the two-phase inflate leaves the
+    first kind to the one-phase kernel and every block comes out as zlib makes it, CRC32 included."""
+    from cubam_amd import _lib
+
+    rng = random.Random(71)
+    vocab = [bytes(rng.choices(range(256), k=3)) for _ in range(6)]
+    payloads, blob = [], b""
+    for i in range(120):
+        kind = i % 4
+        if kind == 0:  # thousands of short matches
+            p = b"".join(rng.choice(vocab) + bytes([rng.randrange(256)]) for _ in range(16000))[:65280]
+            lvl, strat = 1, zlib.Z_DEFAULT_STRATEGY
+        elif kind == 1:
+            p, lvl, strat = _data(rng.randint(100, 65280), seed=i), 0, zlib.Z_DEFAULT_STRATEGY
+        elif kind == 2:
+            p, lvl, strat = _data(rng.randint(100, 65280), seed=i), 6, zlib.Z_FIXED
+        else:
+            p, lvl, strat = _data(rng.randint(100, 65280), seed=i), 9, zlib.Z_DEFAULT_STRATEGY
+        payloads.append(p)
+        blob += bf.bgzf_block(p, lvl, strat)
+    blob += bf.EOF_BLOCK
+    blocks, inf = _gpu_blocks(np.frombuffer(blob, np.uint8), _lib.InflateTune(variant, 0, 0, 0))
+    assert int(inf.result.item()) == 0
+    assert (inf.status.cpu().numpy() == 0).all()
+    assert inf.data[: inf.n].cpu().numpy().tobytes() == b"".join(payloads)
+    crc = inf.crc.cpu().numpy().astype(np.int64).tolist()[: len(payloads)]
+    assert crc == [zlib.crc32(p) for p in payloads]
