@@ -140,17 +140,37 @@ def scan_blocks(buf: torch.Tensor, n: int, eof: bool, ws: Workspace | None = Non
                   isize[:nb], crc[:nb], out_off[: nb + 1], int(h[_lib.SCAN_NCAND]))
 
 
-_tune = _lib.InflateTune(0, 0, 0, 0)
+_tune: _lib.InflateTune | None = None  # None: the autotuned shape, else the library default
 
 
 def set_inflate_tune(variant: int = 0, blocks_per_wave: int = 0, waves_per_group: int = 0, groups: int = 0) -> None:
-    """Launch shape of the inflate kernel for this process (zeros: library defaults)."""
+    """Launch shape of the inflate kernel for this process; all zeros clear it (the autotuned
+    shape where one is cached, else the library default)."""
     global _tune
-    _tune = _lib.InflateTune(variant, blocks_per_wave, waves_per_group, groups)
+    shape = (variant, blocks_per_wave, waves_per_group, groups)
+    _tune = _lib.InflateTune(*shape) if any(shape) else None
 
 
 def get_inflate_tune() -> tuple[int, int, int, int]:
-    return (_tune.variant, _tune.blocks_per_wave, _tune.waves_per_group, _tune.groups)
+    t = _tune
+    return (0, 0, 0, 0) if t is None else (t.variant, t.blocks_per_wave, t.waves_per_group, t.groups)
+
+
+def inflate_tune_for(n_blocks: int, inflated: int, tune: _lib.InflateTune | None = None) -> _lib.InflateTune | None:
+    """The launch shape of an inflate of this size: ``tune``, else the process's, else the
+    autotuned one cached for the size, else None (the library default)."""
+    t = tune if tune is not None else _tune
+    if t is None:
+        from cubam_amd import tune as autotuner
+
+        if autotuner.enabled():
+            t = autotuner.lookup(n_blocks, inflated)
+    return t
+
+
+def tune_arg(t: _lib.InflateTune | None):
+    """The cbm_inflate tune argument: a pointer to ``t``, or null for the library default."""
+    return None if t is None else ctypes.byref(t)
 
 
 @dataclass
@@ -177,17 +197,13 @@ def inflate_blocks(blocks: Blocks, prefix: torch.Tensor | None = None, ws: Works
     crc = torch.empty(max(nb, 1), dtype=torch.uint32, device=dev)
     status = torch.empty(max(nb, 1), dtype=_I32, device=dev)
     result = torch.empty(1, dtype=_I64, device=dev)
-    t = tune if tune is not None else _tune
-    if tune is None and not any(get_inflate_tune()):
-        from cubam_amd import tune as autotuner
-
-        if autotuner.enabled():
-            t = autotuner.lookup(nb, blocks.total_out) or t
+    t = inflate_tune_for(nb, blocks.total_out, tune)
     iws = ws.dev("inflate_ws", L.cbm_inflate_workspace(nb))
     check(
         L.cbm_inflate(
             ptr(blocks.buf), ptr(blocks.blk_off), ptr(blocks.bsize), ptr(blocks.isize), ptr(blocks.crc),
-            ptr(blocks.out_off), nb, data.data_ptr() + p, ptr(crc), ptr(status), ptr(result), ctypes.byref(t),
+            ptr(blocks.out_off), nb, data.data_ptr() + p, ptr(crc), ptr(status), ptr(result),
+            tune_arg(t),
             ptr(iws), iws.numel(), stream_ptr(),
         ),
         "cbm_inflate",

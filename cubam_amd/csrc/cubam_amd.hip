@@ -228,11 +228,25 @@ static int64_t tok_ws_bytes(int64_t n) { return ((12 * n + 255) / 256) * 256 + n
 // dst, matches to the block's list), the resolve phase (the matches, in dst), the CRC32 of every
 // block the decode phase finished, then the one-phase kernel over the blocks it left.
 // ws: the one-phase workspace (counter, statistics, max_groups match lists), then tok_ws_bytes(n).
+// the resolve phase: MPL matches a lane, SPAN-byte windows, WPG blocks a workgroup
+template <int MPL, int SPAN, int WPG>
+static int launch_resolve(uint8_t* dst, const int64_t* out_off, const int32_t* isize, const uint32_t* crc_expect,
+                          uint32_t* crc, int32_t* status, int64_t* result, const uint64_t* pool, const int32_t* mcount,
+                          int64_t n_blocks, hipStream_t st) {
+    // the workgroups that fit a CU's LDS
+    const int64_t per_cu = (160 * 1024) / (int64_t)((sizeof(ResSmem<SPAN, WPG>) + 511) / 512 * 512);
+    int64_t g = (n_blocks + WPG - 1) / WPG;
+    if (g > (int64_t)device_cus() * per_cu) g = (int64_t)device_cus() * per_cu;
+    hipLaunchKernelGGL((inflate_resolve<MPL, SPAN, WPG>), dim3((unsigned)g), dim3(WPG * CBM_WAVE), 0, st, dst, out_off,
+                       isize, crc_expect, crc, status, result, pool, mcount, n_blocks);
+    return check_launch("inflate_resolve");
+}
+
 template <int SEG>
 static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                       const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
-                      uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, void* ws, int64_t ws_groups,
-                      int64_t max_groups, hipStream_t st) {
+                      uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int res_mpl, int res_wpg,
+                      void* ws, int64_t ws_groups, int64_t max_groups, hipStream_t st) {
     int64_t* counter = (int64_t*)ws;  // [0] decode-phase next block, [1] one-phase next block, [2] blocks left
     uint64_t* mlists = (uint64_t*)((uint8_t*)ws + INFLATE_WS_HDR);
     uint8_t* tok = (uint8_t*)mlists + ws_groups * (int64_t)MLIST_CAP * 8;
@@ -243,13 +257,24 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
                                                 status, result, groups, (int64_t)1 << 40, counter, nullptr,
                                                 counter + 8, nullptr, nullptr, pool, mcount, st);
     if (rc) return rc;
-    // the resolve workgroups that fit a CU's LDS
-    const int64_t res_per_cu = (160 * 1024) / (int64_t)((sizeof(ResSmem) + 511) / 512 * 512);
-    int64_t g = (n_blocks + RES_WAVES - 1) / RES_WAVES;
-    if (g > (int64_t)device_cus() * res_per_cu) g = (int64_t)device_cus() * res_per_cu;
-    hipLaunchKernelGGL(inflate_resolve, dim3((unsigned)g), dim3(RES_NT), 0, st, dst, out_off, isize, crc_expect, crc,
-                       status, result, pool, mcount, n_blocks);
-    if ((rc = check_launch("inflate_resolve"))) return rc;
+    switch (res_mpl * 100 + res_wpg) {
+#define RES_CASE(MPL_, WPG_)                                                                                     \
+    case MPL_ * 100 + WPG_:                                                                                      \
+        rc = launch_resolve<MPL_, 2048 * MPL_, WPG_>(dst, out_off, isize, crc_expect, crc, status, result, pool, \
+                                                     mcount, n_blocks, st);                                      \
+        break;
+        RES_CASE(1, 4)
+        RES_CASE(1, 2)
+        RES_CASE(1, 8)
+        RES_CASE(2, 4)
+        RES_CASE(2, 2)
+        RES_CASE(4, 4)
+#undef RES_CASE
+        default:
+            set_error("cbm_inflate: two-phase resolve shape (blocks_per_wave, waves_per_group) not built");
+            return CBM_ERR_ARG;
+    }
+    if (rc) return rc;
     int64_t lg = (n_blocks + 255) / 256;
     if (lg > 1024) lg = 1024;
     hipLaunchKernelGGL(inflate_defer_list, dim3((unsigned)lg), dim3(256), 0, st, status, n_blocks, idx, counter + 2);
@@ -362,6 +387,10 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
     int variant = tune ? tune->variant : CBM_INFLATE_DEFAULT;
     if (tok_variant(variant) && !tok) variant = CBM_INFLATE_WG512_S128;
     const int64_t groups = tune ? tune->groups : 0;
+    // two-phase variants: blocks_per_wave = matches a lane of the resolve phase, waves_per_group =
+    // its blocks a workgroup
+    const int res_mpl = tune && tune->blocks_per_wave > 0 ? tune->blocks_per_wave : 1;
+    const int res_wpg = tune && tune->waves_per_group > 0 ? tune->waves_per_group : 4;
     if (variant != CBM_INFLATE_LANES) {
         int64_t* counter = (int64_t*)ws;  // [0] next block, [8..] decoder statistics
         if ((rc = fill(counter, 8 + STAT_N, 0, st))) return rc;
@@ -384,7 +413,7 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
 #define TOK_CASE(V, SEG_)                                                                                    \
     case V:                                                                                                  \
         return launch_tok<SEG_>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, \
-                                result, groups, ws, inflate_max_groups(), max_groups, st);
+                                result, groups, res_mpl, res_wpg, ws, inflate_max_groups(), max_groups, st);
             TOK_CASE(CBM_INFLATE_TOK64_S512, 512)
             TOK_CASE(CBM_INFLATE_TOK64_S1024, 1024)
             TOK_CASE(CBM_INFLATE_TOK64_S256, 256)

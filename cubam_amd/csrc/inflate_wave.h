@@ -651,6 +651,9 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
         if (kb >= (n_dev ? *n_dev : n_blocks)) break;
         const int64_t b = idx ? idx[kb] : kb;
         uint64_t* blist = TOK ? pool + (size_t)b * MCAP_TOK : nullptr;  // this block's match list (TOK)
+        // this block's output, read once: a store through dst may alias out_off, so a read inside
+        // a store loop would be repeated (and waited for) at every store
+        uint8_t* const bdst = dst + out_off[b];
         const uint8_t* in = src + blk_off[b] + 18;
         const int in_bytes = bsize[b] - 26;
         const int in_bits = 8 * in_bytes;
@@ -717,7 +720,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
             if (type == 0) {
                 const int len = s.sv[SV_LEN], sp = s.sv[SV_SRC], op = s.sv[SV_OP];
                 if constexpr (TOK) {
-                    uint8_t* d = dst + out_off[b] + op;
+                    uint8_t* d = bdst + op;
                     for (int i = tid; i < len; i += NT) d[i] = in[sp + i];
                 } else {
                     for (int i = tid; i < len; i += NT) s.out[op + i] = in[sp + i];
@@ -1094,7 +1097,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                         const uint64_t mrec = (uint64_t)(uint32_t)o | ((uint64_t)(uint32_t)y.dist << 17) |
                                               ((uint64_t)(uint32_t)y.val << 33);
                         if constexpr (TOK) {
-                            if (y.kind == 0) dst[out_off[b] + o] = (uint8_t)y.val;
+                            if (y.kind == 0) bdst[o] = (uint8_t)y.val;
                             else blist[k++] = mrec;
                         } else {
                             if (y.kind == 0) s.out[o] = (uint8_t)y.val;
@@ -1107,7 +1110,9 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                         s.sv[SV_SITE] = SITE_DIST;
                     }
                 }
-                __threadfence_block();
+                // the pass's LDS writes before pass D reads them (TOK: its global writes are read
+                // only by later launches)
+                if constexpr (!TOK) __threadfence_block();
                 __syncthreads();
                 if (s.sv[SV_ERR] != CBM_BLOCK_OK) break;
                 PHASE(STAT_PASS_C);
@@ -1330,6 +1335,10 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
             if (tid == 0) {
                 status[b] = err == CBM_BLOCK_OK ? CBM_BLOCK_OK : BLOCK_DEFER;
                 mcount[b] = s.sv[SV_MCNT];
+#if CBM_INFLATE_STATS
+                if (stats)
+                    for (int k = 0; k < STAT_N; k++) atomicAdd(&stats[k], (unsigned long long)acc[k]), acc[k] = 0;
+#endif
             }
             __syncthreads();
             continue;
@@ -1403,24 +1412,28 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
 }
 
 // ------------------------------------------------------------------ resolve phase (two-phase inflate)
-// A wavefront per block that the decode phase finished (status OK), then the block's CRC32.
-// The block's matches (in output order) go in windows of up to 64, a match a lane, whose output
-// span [lo, hi) fits RES_SPAN bytes: the span is staged in LDS from dst (its literals are in
-// place), the bytes the window's matches write are marked pending in a bitmap, and each step
-// applies every match whose source holds no pending byte (bytes before lo are final in dst:
-// earlier windows were stored); then the span goes back to dst. The lowest pending match always
-// has a final source, so every step applies one at least. A match reads only final bytes,
+// A wavefront per block that the decode phase finished (status OK), then the block's CRC32;
+// WPG blocks a workgroup. The block's matches (in output order) go in windows of up to 64 * MPL,
+// MPL a lane, whose output span [lo, hi) fits SPAN bytes: the span is staged in LDS from dst (its
+// literals are in place; LDS byte shift + p - lo holds byte p, so 16-byte global accesses are
+// 16-byte LDS accesses). Far matches (their source wholly before the span, so final in dst, and
+// not overlapping) are copied at once, each by its lane. The others are marked pending in a
+// bitmap, and each step applies every one whose source holds no pending byte: the lowest pending
+// match always qualifies. Then the span goes back to dst. A match reads only final bytes,
 // [src, src + len) or the dist bytes before dest that an overlapping copy repeats.
-constexpr int RES_NT = 256;
-constexpr int RES_WAVES = RES_NT / CBM_WAVE;
-constexpr int RES_SPAN = 4096;
+constexpr int RES_MAX_DIST = 32768;
 
+template <int SPAN, int WPG>
 struct ResSmem {
     uint32_t crc_t[4][256];
     uint32_t crc_x[64];
-    alignas(16) uint8_t span[RES_WAVES][RES_SPAN + 16];
-    uint32_t pend[RES_WAVES][RES_SPAN / 32 + 2];
+    alignas(16) uint8_t span[WPG][SPAN + 48];
+    uint32_t pend[WPG][(SPAN + 16) / 32 + 2];
 };
+
+// LDS order among the lanes of one wavefront: the wavefront's LDS operations run in program
+// order, so a compiler barrier is all it takes (no wait on global memory)
+__device__ __forceinline__ void res_wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
 __device__ __forceinline__ void res_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1477,80 +1490,135 @@ __device__ __forceinline__ void res_copy_far(const uint8_t* g, uint8_t* d, int l
     }
 }
 
-__global__ void __launch_bounds__(RES_NT) inflate_resolve(uint8_t* dst, const int64_t* out_off, const int32_t* isize,
-                                                          const uint32_t* expect, uint32_t* crc_out, int32_t* status,
-                                                          int64_t* bad, const uint64_t* pool, const int32_t* mcount,
-                                                          int64_t n_blocks) {
-    __shared__ ResSmem s;
+template <int MPL, int SPAN, int WPG>
+__global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve(uint8_t* dst, const int64_t* out_off,
+                                                                  const int32_t* isize, const uint32_t* expect,
+                                                                  uint32_t* crc_out, int32_t* status, int64_t* bad,
+                                                                  const uint64_t* pool, const int32_t* mcount,
+                                                                  int64_t n_blocks) {
+    static_assert(SPAN >= 258 && SPAN % 32 == 0, "a window holds one match at least");
+    __shared__ ResSmem<SPAN, WPG> s;
     crc_tables(s.crc_t, s.crc_x);
     const int wv = (int)threadIdx.x / CBM_WAVE, lane = (int)threadIdx.x % CBM_WAVE;
     uint8_t* sp = s.span[wv];
     uint32_t* pend = s.pend[wv];
-    for (int64_t b = (int64_t)blockIdx.x * RES_WAVES + wv; b < n_blocks; b += (int64_t)gridDim.x * RES_WAVES) {
+    for (int64_t b = (int64_t)blockIdx.x * WPG + wv; b < n_blocks; b += (int64_t)gridDim.x * WPG) {
         if (status[b] != CBM_BLOCK_OK) continue;  // left to the one-phase kernel
         const int M = mcount[b];
         uint8_t* out = dst + out_off[b];
         const uint64_t* ml = pool + (size_t)b * MCAP_TOK;
         bool stuck = false;
         for (int k0 = 0; k0 < M && !stuck;) {
-            const int k = k0 + lane;
-            const uint64_t m = k < M ? ml[k] : 0;
-            const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff), len = (int)((m >> 33) & 0x1ff);
-            const int lo = __builtin_amdgcn_readlane(dest, 0);
-            // dest + len grows with k (outputs are disjoint and in order): the lanes that fit are a prefix
-            const uint64_t fit = __ballot(k < M && dest + len - lo <= RES_SPAN);
-            const int c = __popcll(fit);  // lane 0's match (at most 258 bytes) always fits
-            const bool mine = lane < c;
-            const int hi = __builtin_amdgcn_readlane(dest + len, c - 1);
+            int dest[MPL], dist[MPL], len[MPL];
+#pragma unroll
+            for (int u = 0; u < MPL; u++) {
+                const int k = k0 + u * CBM_WAVE + lane;
+                const uint64_t m = k < M ? ml[k] : 0;
+                dest[u] = (int)(m & 0x1ffff), dist[u] = (int)((m >> 17) & 0xffff), len[u] = (int)((m >> 33) & 0x1ff);
+            }
+            const int lo = __builtin_amdgcn_readlane(dest[0], 0);
+            // dest + len grows with the match index (outputs are disjoint and in order): the
+            // matches that fit are a prefix, c of them (match k0's at most 258 bytes always fits)
+            int c = 0;
+#pragma unroll
+            for (int u = 0; u < MPL; u++)
+                c += __popcll(__ballot(k0 + u * CBM_WAVE + lane < M && dest[u] + len[u] - lo <= SPAN));
+            int hi = 0;
+#pragma unroll
+            for (int u = 0; u < MPL; u++)
+                if ((c - 1) / CBM_WAVE == u) hi = __builtin_amdgcn_readlane(dest[u] + len[u], (c - 1) % CBM_WAVE);
             const int nspan = hi - lo;
-            for (int i = lane; i < nspan; i += CBM_WAVE) sp[i] = out[lo + i];
-            for (int w = lane; w <= (nspan >> 5); w += CBM_WAVE) pend[w] = 0;
-            const int src = dest - dist, srcn = dist < len ? dist : len;
-            // far matches (the source wholly before the span, so final in dst, and no overlap) need
-            // nothing of the window: each lane copies its own first, the loads of a pass together
-            const bool far = mine && dist >= len && src + len <= lo;
-            if (far) res_copy_far(out + src, sp + (dest - lo), len);
-            __threadfence_block();
-            // the others wait on the bytes the window writes
-            const bool near = mine && !far;
-            if (near) mark_bits(pend, dest - lo, len, true);
-            __threadfence_block();
-            bool left = near;
+            const int shift = (int)((uintptr_t)(out + lo) & 15);
+            // stage the span: the 16-byte aligned pieces that cover it
+            const uint8_t* g0 = out + lo - shift;
+            const int npiece = (shift + nspan + 15) >> 4;
+            for (int q = lane; q < npiece; q += CBM_WAVE) *(uint4*)(sp + 16 * q) = *(const uint4*)(g0 + 16 * q);
+            for (int w = lane; w <= ((shift + nspan) >> 5); w += CBM_WAVE) pend[w] = 0;
+            bool near[MPL];
+#pragma unroll
+            for (int u = 0; u < MPL; u++) {
+                const bool mine = u * CBM_WAVE + lane < c;
+                const int src = dest[u] - dist[u];
+                // far: needs nothing of the window, so its lane copies it at once
+                const bool far = mine && dist[u] >= len[u] && src + len[u] <= lo;
+                if (far) res_copy_far(out + src, sp + shift + dest[u] - lo, len[u]);
+                near[u] = mine && !far;
+            }
+            res_wave_fence();
+#pragma unroll
+            for (int u = 0; u < MPL; u++)
+                if (near[u]) mark_bits(pend, shift + dest[u] - lo, len[u], true);
+            res_wave_fence();
+            uint32_t left = 0;
+#pragma unroll
+            for (int u = 0; u < MPL; u++) left |= near[u] ? 1u << u : 0u;
             int idle = 0;
-            while (__ballot(left)) {
-                bool ready = false;
-                if (left) {
-                    const int a = src < lo ? lo : src, e = src + srcn;
-                    ready = !(a < e && any_bits(pend, a - lo, e - lo));
+            while (__ballot(left != 0)) {
+                uint32_t rdy = 0;
+#pragma unroll
+                for (int u = 0; u < MPL; u++) {
+                    if (!(left & (1u << u))) continue;
+                    const int src = dest[u] - dist[u], e = src + (dist[u] < len[u] ? dist[u] : len[u]);
+                    const int a0 = src < lo ? lo : src;
+                    if (!(a0 < e && any_bits(pend, shift + a0 - lo, shift + e - lo))) rdy |= 1u << u;
                 }
-                if (!__ballot(ready) && ++idle > (1 << 12)) {  // cannot happen (the lowest is ready):
-                    stuck = true;                               // leave the block to the one-phase kernel
+                if (!__ballot(rdy != 0) && ++idle > (1 << 12)) {  // cannot happen (the lowest is ready):
+                    stuck = true;                                  // leave the block to the one-phase kernel
                     break;
                 }
-                if (ready) {
+#pragma unroll
+                for (int u = 0; u < MPL; u++) {
+                    if (!(rdy & (1u << u))) continue;
                     // eight bytes a pass, loads before stores: every byte read is final (an
                     // overlapping copy repeats [src, dest), r = i mod dist)
+                    const int src = dest[u] - dist[u], dl = dist[u], ln = len[u];
+                    uint8_t* d = sp + shift + dest[u] - lo;
                     int r = 0;
-                    for (int i0 = 0; i0 < len; i0 += 8) {
-                        uint8_t t[8];
+                    if (src >= lo) {  // the source is in the span: LDS only
+                        const uint8_t* sl = sp + shift + src - lo;
+                        for (int i0 = 0; i0 < ln; i0 += 8) {
+                            uint8_t t[8];
 #pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            const int i = i0 + j;
-                            const int q = src + (dist >= len ? i : r);
-                            t[j] = i < len ? (q < lo ? out[q] : sp[q - lo]) : 0;
-                            r = r + 1 == dist ? 0 : r + 1;
+                            for (int j = 0; j < 8; j++) {
+                                t[j] = i0 + j < ln ? sl[dl >= ln ? i0 + j : r] : 0;
+                                r = r + 1 == dl ? 0 : r + 1;
+                            }
+#pragma unroll
+                            for (int j = 0; j < 8; j++)
+                                if (i0 + j < ln) d[i0 + j] = t[j];
                         }
+                    } else {  // partly before the span (in dst), partly in it
+                        for (int i0 = 0; i0 < ln; i0 += 8) {
+                            uint8_t t[8];
 #pragma unroll
-                        for (int j = 0; j < 8; j++)
-                            if (i0 + j < len) sp[dest - lo + i0 + j] = t[j];
+                            for (int j = 0; j < 8; j++) {
+                                const int i = i0 + j;
+                                const int q = src + (dl >= ln ? i : r);
+                                t[j] = i < ln ? (q < lo ? out[q] : sp[shift + q - lo]) : 0;
+                                r = r + 1 == dl ? 0 : r + 1;
+                            }
+#pragma unroll
+                            for (int j = 0; j < 8; j++)
+                                if (i0 + j < ln) d[i0 + j] = t[j];
+                        }
                     }
                 }
-                __threadfence_block();  // the bytes land before their bits clear
-                if (ready) mark_bits(pend, dest - lo, len, false);
-                __threadfence_block();
-                left = left && !ready;
+                res_wave_fence();  // the bytes land before their bits clear
+#pragma unroll
+                for (int u = 0; u < MPL; u++)
+                    if (rdy & (1u << u)) mark_bits(pend, shift + dest[u] - lo, len[u], false);
+                res_wave_fence();
+                left &= ~rdy;
             }
-            for (int i = lane; i < nspan; i += CBM_WAVE) out[lo + i] = sp[i];
+            // the span back to dst: single bytes up to a 16-byte boundary, 16-byte pieces, single bytes
+            {
+                const int head = ((16 - shift) & 15) < nspan ? ((16 - shift) & 15) : nspan;
+                if (lane < head) out[lo + lane] = sp[shift + lane];
+                const int body = (nspan - head) >> 4;
+                for (int q = lane; q < body; q += CBM_WAVE)
+                    *(uint4*)(out + lo + head + 16 * q) = *(const uint4*)(sp + shift + head + 16 * q);
+                for (int i = head + 16 * body + lane; i < nspan; i += CBM_WAVE) out[lo + i] = sp[shift + i];
+            }
             res_fence();  // later windows read these bytes from dst
             k0 += c;
         }
@@ -1561,11 +1629,11 @@ __global__ void __launch_bounds__(RES_NT) inflate_resolve(uint8_t* dst, const in
         // CRC32: lane j takes the j-th of 64 stretches of 1,024 bytes counted back from the end
         const int L = isize[b];
         const int e = L - (CBM_WAVE - 1 - lane) * 1024, st = e - 1024 > 0 ? e - 1024 : 0;
-        uint32_t c = e > st ? crc_mulmod(s.crc_x[CBM_WAVE - 1 - lane], crc_bytes16(out + st, e - st, s.crc_t)) : 0;
-        for (int d = 32; d; d >>= 1) c ^= __shfl_xor(c, d, CBM_WAVE);
+        uint32_t cr = e > st ? crc_mulmod(s.crc_x[CBM_WAVE - 1 - lane], crc_bytes16(out + st, e - st, s.crc_t)) : 0;
+        for (int d = 32; d; d >>= 1) cr ^= __shfl_xor(cr, d, CBM_WAVE);
         if (lane == 0) {
-            crc_out[b] = c;
-            if (c != expect[b]) {
+            crc_out[b] = cr;
+            if (cr != expect[b]) {
                 status[b] = CBM_BLOCK_CRC;
                 atomicAdd((unsigned long long*)bad, 1ull);
             }

@@ -240,10 +240,10 @@ def decode_staged(staged: Staged, tags=("CB", "UB"), ws: _decode.Workspace | Non
     crc_out = torch.empty(max(NB, 1), dtype=torch.uint32, device=dev)
     status = torch.empty(max(NB, 1), dtype=_I32, device=dev)
     inf_bad = torch.empty(1, dtype=_I64, device=dev)
-    tune = _lib.InflateTune(*_decode.get_inflate_tune())
+    tune = _decode.inflate_tune_for(NB, total_out)
     iws = ws.dev("inflate_ws", L.cbm_inflate_workspace(NB))
     check(L.cbm_inflate(ptr(staging), ptr(blk_off_d), ptr(bsize_d), ptr(isize_d), crc_d.data_ptr(), ptr(out_off_d),
-                        NB, ptr(data), ptr(crc_out), ptr(status), ptr(inf_bad), ctypes.byref(tune), ptr(iws),
+                        NB, ptr(data), ptr(crc_out), ptr(status), ptr(inf_bad), _decode.tune_arg(tune), ptr(iws),
                         iws.numel(), st), "cbm_inflate")
     # ---- kernel 3 over all spans
     rws = ws.dev("spans_ws", L.cbm_record_spans_workspace(n_seg))

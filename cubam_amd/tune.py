@@ -164,7 +164,8 @@ class GraphInflate:
     def __init__(self, blocks: _decode.Blocks, tune=None, ws: _decode.Workspace | None = None):
         L = _lib.load()
         self.blocks = blocks
-        self.tune = _lib.InflateTune(*(tune or _decode.get_inflate_tune()))
+        self.tune = _decode.inflate_tune_for(blocks.n_blocks, blocks.total_out,
+                                             _lib.InflateTune(*tune) if tune else None)
         dev = blocks.buf.device
         nb = blocks.n_blocks
         self.data = torch.empty(blocks.total_out + _lib.PAD, dtype=torch.uint8, device=dev)
@@ -187,7 +188,7 @@ class GraphInflate:
         _lib.check(L.cbm_inflate(b.buf.data_ptr(), b.blk_off.data_ptr(), b.bsize.data_ptr(), b.isize.data_ptr(),
                                  b.crc.data_ptr(), b.out_off.data_ptr(), b.n_blocks, self.data.data_ptr(),
                                  self.crc.data_ptr(), self.status.data_ptr(), self.bad.data_ptr(),
-                                 ctypes.byref(self.tune), self.ws.data_ptr(), self.ws.numel(), _lib.stream_ptr()),
+                                 _decode.tune_arg(self.tune), self.ws.data_ptr(), self.ws.numel(), _lib.stream_ptr()),
                    "cbm_inflate")
 
     def run(self) -> None:

@@ -84,9 +84,10 @@ enum {
 };
 
 /* Inflate kernel variants (cbm_inflate_tune.variant): a workgroup of NT threads decodes each
-   block, each thread SEG bits of a round; or one block per lane (CBM_INFLATE_LANES). */
+   block, each thread SEG bits of a round; or one block per lane (CBM_INFLATE_LANES); or the
+   two-phase variants. A null tune selects CBM_INFLATE_DEFAULT. */
 enum {
-    CBM_INFLATE_WG512_S128 = 0, /* default: NT 512, SEG 128 */
+    CBM_INFLATE_WG512_S128 = 0, /* NT 512, SEG 128: the one-phase kernel the others fall back to */
     CBM_INFLATE_LANES = 1,      /* blocks_per_wave lanes of a wavefront each inflate a block */
     CBM_INFLATE_WG128 = 2,      /* NT 128, SEG 256 */
     CBM_INFLATE_WG512 = 3,      /* NT 512, SEG 64 */
@@ -104,15 +105,20 @@ enum {
     CBM_INFLATE_TOK64_S1024 = 12,
     CBM_INFLATE_TOK64_S256 = 13,
     CBM_INFLATE_VARIANTS = 14,
-    CBM_INFLATE_DEFAULT = CBM_INFLATE_WG512_S128 /* variant 0 and a null tune */
+    CBM_INFLATE_DEFAULT = CBM_INFLATE_TOK64_S512 /* a null tune */
 };
 
-/* Launch shape of the inflate kernel; zeros select the defaults (or the autotuned choice). */
+/* Launch shape of the inflate kernel; zero fields select the variant's defaults. */
 typedef struct {
-    int32_t variant;         /* CBM_INFLATE_* */
-    int32_t blocks_per_wave; /* CBM_INFLATE_LANES: lanes of a wavefront that each inflate a block, 1..64 */
-    int32_t waves_per_group; /* CBM_INFLATE_LANES: wavefronts per workgroup, 1..16 */
-    int32_t groups;          /* workgroups launched (persistent); 0 = as many as the CUs hold */
+    int32_t variant; /* CBM_INFLATE_* */
+    /* CBM_INFLATE_LANES: lanes of a wavefront that each inflate a block, 1..64 (default 4);
+       two-phase variants: matches a lane of the resolve pass takes per window, 1, 2 or 4 (default 1) */
+    int32_t blocks_per_wave;
+    /* CBM_INFLATE_LANES: wavefronts per workgroup, 1..16 (default 2); two-phase variants: blocks
+       (one a wavefront) per workgroup of the resolve pass, 2, 4 or 8 (default 4; 4 only with 4
+       matches a lane) */
+    int32_t waves_per_group;
+    int32_t groups; /* workgroups of the (first) decode launch; 0 = as many as the CUs hold */
 } cbm_inflate_tune;
 
 /* Bytes of workspace cbm_inflate needs for n_blocks blocks. */
