@@ -68,8 +68,8 @@ def gpu_step(buf, n, start, n_ref, ws, ev=None):
     if ev is not None:
         ev[1].record()
     chain = _decode.record_chain(inf.data, start, inf.n, n_ref, ws, inf=inf)
-    off = _decode.record_offsets(inf.data, start, inf.n, chain)
-    cols = _decode.scatter(inf.data, off, chain, tags=("CB", "UB"), ws=ws)
+    off, csr = _decode.record_offsets(inf.data, start, inf.n, chain, csr=True)
+    cols = _decode.scatter(inf.data, off, chain, tags=("CB", "UB"), ws=ws, csr=csr)
     return inf, chain, off, cols
 
 

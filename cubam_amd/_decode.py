@@ -285,12 +285,26 @@ def record_candidates(data: torch.Tensor, start: int, n: int, n_ref: int, ws: Wo
     return int(cnt[:8].view(_I64)[0].item())
 
 
-def record_offsets(data: torch.Tensor, start: int, n: int, chain: Chain) -> torch.Tensor:
+def record_offsets(data: torch.Tensor, start: int, n: int, chain: Chain, csr: bool = False):
+    """The chain's record offsets; with ``csr``, ``(offsets, csr_columns)``: also the CSR offset
+    columns ``cigar_off``, ``seq_off`` and ``qual_off`` (``n_rec + 1`` each) from the chain's
+    sums, for :func:`scatter`."""
     L = _lib.load()
-    off = torch.empty(max(chain.n_rec, 1), dtype=_I64, device=data.device)
-    check(L.cbm_record_offsets(ptr(data), start, n, ptr(chain.ws), chain.ws.numel(), ptr(off), chain.n_rec,
-                               stream_ptr()), "cbm_record_offsets")
-    return off[: chain.n_rec]
+    dev = data.device
+    off = torch.empty(max(chain.n_rec, 1), dtype=_I64, device=dev)
+    if not csr:
+        check(L.cbm_record_offsets(ptr(data), start, n, ptr(chain.ws), chain.ws.numel(), ptr(off), chain.n_rec,
+                                   stream_ptr()), "cbm_record_offsets")
+        return off[: chain.n_rec]
+    cols = {k: torch.empty(chain.n_rec + 1, dtype=_I64, device=dev) for k in CSR_OFFSETS}
+    check(L.cbm_record_offsets_csr(ptr(data), start, n, ptr(chain.ws), chain.ws.numel(), ptr(off),
+                                   *(ptr(cols[k]) for k in CSR_OFFSETS), chain.n_rec, stream_ptr()),
+          "cbm_record_offsets_csr")
+    return off[: chain.n_rec], cols
+
+
+#: the CSR offset columns, in the order of the C ABI's arguments
+CSR_OFFSETS = ("cigar_off", "seq_off", "qual_off")
 
 
 FIXED_TYPES = {
@@ -300,10 +314,12 @@ FIXED_TYPES = {
 
 
 def scatter(data: torch.Tensor, offsets: torch.Tensor, chain: Chain | None, tags=("CB", "UB"), var: bool = True,
-            fixed: bool = True, ws: Workspace | None = None) -> dict:
+            fixed: bool = True, ws: Workspace | None = None, csr: dict | None = None) -> dict:
     """Kernel 4: the columns of the records at `offsets` (asynchronous).
 
     ``fixed``: the eleven fixed fields; ``var`` (needs the chain's CSR sizes): CIGAR, seq, qual.
+    ``csr``: the CSR offset columns of these records, already computed (``record_offsets(...,
+    csr=True)``); without it they are computed here from the records.
     """
     L = _lib.load()
     ws = ws or default_workspace()
@@ -314,11 +330,12 @@ def scatter(data: torch.Tensor, offsets: torch.Tensor, chain: Chain | None, tags
     for k in cols:
         setattr(c, k, ptr(cols[k]))
     if var:
-        cols["cigar_off"] = torch.empty(n + 1, dtype=_I64, device=dev)
+        for k in CSR_OFFSETS:
+            if csr is not None and csr[k].numel() != n + 1:
+                raise ValueError(f"{k} holds {csr[k].numel()} entries for {n} records")
+            cols[k] = csr[k] if csr is not None else torch.empty(n + 1, dtype=_I64, device=dev)
         cols["cigar"] = torch.empty(max(chain.n_cigar, 1), dtype=torch.uint32, device=dev)
-        cols["seq_off"] = torch.empty(n + 1, dtype=_I64, device=dev)
         cols["seq"] = torch.empty(max(chain.n_seq, 1), dtype=_U8, device=dev)
-        cols["qual_off"] = torch.empty(n + 1, dtype=_I64, device=dev)
         cols["qual"] = torch.empty(max(chain.n_qual, 1), dtype=_U8, device=dev)
         for k in ("cigar_off", "cigar", "seq_off", "seq", "qual_off", "qual"):
             setattr(c, k, ptr(cols[k]))
@@ -340,10 +357,13 @@ def scatter(data: torch.Tensor, offsets: torch.Tensor, chain: Chain | None, tags
         c.tag_type[i], c.tag_offset[i] = ptr(tc["type"]), ptr(tc["offset"])
         c.tag_length[i], c.tag_value[i] = ptr(tc["length"]), ptr(tc["value"])
         tag_cols[t] = tc
-    wsb = L.cbm_scatter_workspace(n)
-    sws = ws.dev("scatter_ws", wsb)
-    check(L.cbm_scatter(ptr(data), ptr(offsets), n, ctypes.byref(c), ptr(sws), sws.numel(), stream_ptr()),
-          "cbm_scatter")
+    if var and csr is not None:
+        check(L.cbm_scatter_csr(ptr(data), ptr(offsets), n, ctypes.byref(c), stream_ptr()), "cbm_scatter_csr")
+    else:
+        wsb = L.cbm_scatter_workspace(n)
+        sws = ws.dev("scatter_ws", wsb)
+        check(L.cbm_scatter(ptr(data), ptr(offsets), n, ctypes.byref(c), ptr(sws), sws.numel(), stream_ptr()),
+              "cbm_scatter")
     if var:
         cols["cigar"] = cols["cigar"][: chain.n_cigar]
         cols["seq"] = cols["seq"][: chain.n_seq]

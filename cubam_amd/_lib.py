@@ -85,12 +85,15 @@ SYMBOLS = (
     "cbm_record_chain_repair",
     "cbm_record_candidates",
     "cbm_record_offsets",
+    "cbm_record_offsets_csr",
     "cbm_record_spans_workspace",
     "cbm_record_chain_spans",
     "cbm_record_chain_spans_repair",
     "cbm_record_offsets_spans",
+    "cbm_record_offsets_spans_csr",
     "cbm_scatter_workspace",
     "cbm_scatter",
+    "cbm_scatter_csr",
     "cbm_name_hash",
     "cbm_hash_spans",
     "cbm_pack_acgt",
@@ -141,6 +144,8 @@ def load():
     L.cbm_record_candidates.argtypes = [vp, i64, i64, i32, vp, vp]
     L.cbm_record_offsets.restype = ctypes.c_int
     L.cbm_record_offsets.argtypes = [vp, i64, i64, vp, i64, vp, i64, vp]
+    L.cbm_record_offsets_csr.restype = ctypes.c_int
+    L.cbm_record_offsets_csr.argtypes = [vp, i64, i64, vp, i64, vp, vp, vp, vp, i64, vp]
     L.cbm_record_spans_workspace.restype = i64
     L.cbm_record_spans_workspace.argtypes = [i64]
     for f in (L.cbm_record_chain_spans, L.cbm_record_chain_spans_repair):
@@ -148,10 +153,14 @@ def load():
         f.argtypes = [vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, i64, vp]
     L.cbm_record_offsets_spans.restype = ctypes.c_int
     L.cbm_record_offsets_spans.argtypes = [vp, i64, vp, i64, vp, i64, vp]
+    L.cbm_record_offsets_spans_csr.restype = ctypes.c_int
+    L.cbm_record_offsets_spans_csr.argtypes = [vp, i64, vp, i64, vp, vp, vp, vp, i64, vp]
     L.cbm_scatter_workspace.restype = i64
     L.cbm_scatter_workspace.argtypes = [i64]
     L.cbm_scatter.restype = ctypes.c_int
     L.cbm_scatter.argtypes = [vp, vp, i64, ctypes.POINTER(Columns), vp, i64, vp]
+    L.cbm_scatter_csr.restype = ctypes.c_int
+    L.cbm_scatter_csr.argtypes = [vp, vp, i64, ctypes.POINTER(Columns), vp]
     for f, at in ((L.cbm_name_hash, [vp, vp, i64, vp, vp]), (L.cbm_hash_spans, [vp, vp, vp, i64, vp, vp]),
                   (L.cbm_pack_acgt, [vp, vp, vp, i64, vp, vp, vp]), (L.cbm_aligned_length, [vp, vp, i64, vp, vp]),
                   (L.cbm_blocks, [vp, vp, i64, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp]),

@@ -284,6 +284,35 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
                                                 counter + 2, nullptr, nullptr, st);
 }
 
+// The chain's totals and per-segment bases: exclusive scans over the segments of the record
+// counts (w.base), n_cigar_op sums, base-byte sums and l_seq sums (walks off the chain count 0),
+// their totals into the result slots.
+static int chain_totals(const RecWs& w, int64_t* result, hipStream_t st) {
+    const int64_t* none = nullptr;
+    int rc;
+    if ((rc = exclusive_scan(SegVal{w.f, w.cnt}, StoreI64{w.base}, none, w.nseg, w.scan_ws, result + CBM_REC_N, st)))
+        return rc;
+    if ((rc = exclusive_scan(SegVal{w.f, w.cig}, StoreI64{w.cig_base}, none, w.nseg, w.scan_ws,
+                             result + CBM_REC_CIGAR, st)))
+        return rc;
+    if ((rc = exclusive_scan(SegVal{w.f, w.seqb}, StoreI64{w.seq_base}, none, w.nseg, w.scan_ws,
+                             result + CBM_REC_SEQ, st)))
+        return rc;
+    return exclusive_scan(SegVal{w.f, w.qual}, StoreI64{w.qual_base}, none, w.nseg, w.scan_ws,
+                          result + CBM_REC_QUAL, st);
+}
+
+// the offsets of the first n_rec records of a chain and, for non-null columns, their CSR offsets
+static int chain_emit(const uint8_t* data, int64_t start, int64_t n, const RecWs& w, int64_t* offsets, int64_t n_rec,
+                      CsrOut csr, hipStream_t st) {
+    unsigned g = (unsigned)((w.nseg + EMIT_SEGS - 1) / EMIT_SEGS);
+    hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, st, data, start, n, w, offsets, n_rec, csr);
+    int rc = check_launch("rec_emit");
+    if (rc || !(csr.cig || csr.seq || csr.qual)) return rc;
+    hipLaunchKernelGGL(rec_csr_totals, dim3(1), dim3(1), 0, st, w, n_rec, csr);
+    return check_launch("rec_csr_totals");
+}
+
 }  // namespace cbm
 
 using namespace cbm;
@@ -546,8 +575,7 @@ int cbm_record_chain(const uint8_t* data, int64_t start, int64_t n, int32_t n_re
     if ((rc = check_launch("rec_sum"))) return rc;
     hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
     if ((rc = check_launch("rec_result_links"))) return rc;
-    return exclusive_scan(LoadI64{w.cnt}, StoreI64{w.base}, (const int64_t*)nullptr, w.nseg, w.scan_ws,
-                          (int64_t*)nullptr, st);
+    return chain_totals(w, result, st);
 }
 
 int cbm_record_chain_repair(const uint8_t* data, int64_t start, int64_t n, int32_t n_ref, int64_t* result,
@@ -575,9 +603,7 @@ int cbm_record_chain_repair(const uint8_t* data, int64_t start, int64_t n, int32
     if ((rc = check_launch("rec_zero_dropped"))) return rc;
     hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
     if ((rc = check_launch("rec_result_links"))) return rc;
-    rc = exclusive_scan(LoadI64{w.cnt}, StoreI64{w.base}, (const int64_t*)nullptr, ns, w.scan_ws,
-                        (int64_t*)nullptr, st);
-    if (rc) return rc;
+    if ((rc = chain_totals(w, result, st))) return rc;
     return hipStreamSynchronize(st) == hipSuccess ? CBM_OK : CBM_ERR_LAUNCH;
 }
 
@@ -598,22 +624,31 @@ int cbm_record_candidates(const uint8_t* data, int64_t start, int64_t n, int32_t
 
 int cbm_record_offsets(const uint8_t* data, int64_t start, int64_t n, const void* ws, int64_t ws_bytes,
                        int64_t* offsets, int64_t n_rec, void* stream) {
+    return cbm_record_offsets_csr(data, start, n, ws, ws_bytes, offsets, nullptr, nullptr, nullptr, n_rec, stream);
+}
+
+int cbm_record_offsets_csr(const uint8_t* data, int64_t start, int64_t n, const void* ws, int64_t ws_bytes,
+                           int64_t* offsets, int64_t* cigar_off, int64_t* seq_off, int64_t* qual_off, int64_t n_rec,
+                           void* stream) {
+    hipStream_t st = (hipStream_t)stream;
     ARG(ws && start >= 0 && n >= start && n_rec >= 0, "cbm_record_offsets: bad arguments");
     ARG(ws_bytes >= rec_ws_bytes(n - start), "cbm_record_offsets: workspace too small");
-    if (n_rec == 0) return CBM_OK;
+    const CsrOut csr{cigar_off, seq_off, qual_off};
+    if (n_rec == 0) {  // the CSR columns hold one zero each
+        int rc = CBM_OK;
+        for (int64_t* c : {cigar_off, seq_off, qual_off})
+            if (c && (rc = fill(c, 1, 0, st))) return rc;
+        return rc;
+    }
     ARG(data && offsets, "cbm_record_offsets: null pointer");
-    RecWs w = rec_ws_layout((void*)ws, n - start);
-    unsigned g = (unsigned)((w.nseg + EMIT_SEGS - 1) / EMIT_SEGS);
-    hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, start, n, w, offsets, n_rec);
-    return check_launch("rec_emit");
+    return chain_emit(data, start, n, rec_ws_layout((void*)ws, n - start), offsets, n_rec, csr, st);
 }
 
 // ------------------------------------------------------------------ kernel 3 over many spans
 int64_t cbm_record_spans_workspace(int64_t n_seg) { return n_seg < 0 ? -1 : rec_ws_bytes_seg(n_seg); }
 
-static int spans_finish(SpanTab t, RecWs w, int64_t* span_rec, hipStream_t st) {
-    int rc = exclusive_scan(LoadI64{w.cnt}, StoreI64{w.base}, (const int64_t*)nullptr, w.nseg, w.scan_ws,
-                            (int64_t*)nullptr, st);
+static int spans_finish(SpanTab t, RecWs w, int64_t* result, int64_t* span_rec, hipStream_t st) {
+    int rc = chain_totals(w, result, st);
     if (rc) return rc;
     unsigned g = (unsigned)((t.n + 1 + 255) / 256);
     hipLaunchKernelGGL(recs_bases, dim3(g), dim3(256), 0, st, t, w, span_rec);
@@ -646,7 +681,7 @@ int cbm_record_chain_spans(const uint8_t* data, const int64_t* span_start, const
         hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
         if ((rc = check_launch("rec_result_links"))) return rc;
     }
-    return spans_finish(t, w, span_rec, st);
+    return spans_finish(t, w, result, span_rec, st);
 }
 
 int cbm_record_chain_spans_repair(const uint8_t* data, const int64_t* span_start, const int64_t* span_end,
@@ -685,21 +720,30 @@ int cbm_record_chain_spans_repair(const uint8_t* data, const int64_t* span_start
     if ((rc = check_launch("rec_zero_dropped"))) return rc;
     hipLaunchKernelGGL(rec_result_links, dim3(1), dim3(1), 0, st, result, w);
     if ((rc = check_launch("rec_result_links"))) return rc;
-    if ((rc = spans_finish(t, w, span_rec, st))) return rc;
+    if ((rc = spans_finish(t, w, result, span_rec, st))) return rc;
     return hipStreamSynchronize(st) == hipSuccess ? CBM_OK : CBM_ERR_LAUNCH;
 }
 
 int cbm_record_offsets_spans(const uint8_t* data, int64_t n_seg, const void* ws, int64_t ws_bytes, int64_t* offsets,
                              int64_t n_rec, void* stream) {
+    return cbm_record_offsets_spans_csr(data, n_seg, ws, ws_bytes, offsets, nullptr, nullptr, nullptr, n_rec, stream);
+}
+
+int cbm_record_offsets_spans_csr(const uint8_t* data, int64_t n_seg, const void* ws, int64_t ws_bytes,
+                                 int64_t* offsets, int64_t* cigar_off, int64_t* seq_off, int64_t* qual_off,
+                                 int64_t n_rec, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
     ARG(ws && n_seg >= 0 && n_rec >= 0, "cbm_record_offsets_spans: bad arguments");
     ARG(ws_bytes >= rec_ws_bytes_seg(n_seg), "cbm_record_offsets_spans: workspace too small");
-    if (n_rec == 0 || n_seg == 0) return CBM_OK;
+    if (n_rec == 0 || n_seg == 0) {
+        int rc = CBM_OK;
+        for (int64_t* c : {cigar_off, seq_off, qual_off})
+            if (c && (rc = fill(c, 1, 0, st))) return rc;
+        return rc;
+    }
     ARG(data && offsets, "cbm_record_offsets_spans: null pointer");
-    RecWs w = rec_ws_layout_seg((void*)ws, n_seg);
-    unsigned g = (unsigned)((n_seg + EMIT_SEGS - 1) / EMIT_SEGS);
-    hipLaunchKernelGGL(rec_emit, dim3(g), dim3(REC_NT), 0, (hipStream_t)stream, data, (int64_t)0, (int64_t)0, w,
-                       offsets, n_rec);
-    return check_launch("rec_emit");
+    return chain_emit(data, 0, 0, rec_ws_layout_seg((void*)ws, n_seg), offsets, n_rec,
+                      CsrOut{cigar_off, seq_off, qual_off}, st);
 }
 
 // ------------------------------------------------------------------ on-demand record kernels
@@ -837,6 +881,20 @@ int cbm_scatter(const uint8_t* data, const int64_t* offsets, int64_t n_rec, cons
         if ((rc = check_launch("rec_columns"))) return rc;
     }
     return CBM_OK;
+}
+
+int cbm_scatter_csr(const uint8_t* data, const int64_t* offsets, int64_t n_rec, const cbm_columns* cols,
+                    void* stream) {
+    ARG(n_rec >= 0 && cols, "cbm_scatter_csr: bad arguments");
+    ARG(cols->n_tags >= 0 && cols->n_tags <= 4, "cbm_scatter_csr: at most 4 tags");
+    ARG((!cols->cigar || cols->cigar_off) && (!cols->seq || cols->seq_off) && (!cols->qual || cols->qual_off),
+        "cbm_scatter_csr: a CSR column needs its offsets");
+    if (n_rec == 0) return CBM_OK;
+    ARG(data && offsets, "cbm_scatter_csr: null pointer");
+    unsigned g = (unsigned)((n_rec + FIELDS_NT - 1) / FIELDS_NT);
+    hipLaunchKernelGGL(rec_columns, dim3(g), dim3(FIELDS_THREADS), 0, (hipStream_t)stream, data, offsets, n_rec,
+                       *cols);
+    return check_launch("rec_columns");
 }
 
 }  // extern "C"

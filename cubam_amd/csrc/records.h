@@ -39,6 +39,10 @@ struct RecWs {
     int64_t* hdr;   // [0] bad links [1..7] spare
     int64_t* nrel;  // record starts the walk recorded in rel, or -1 (rec_emit walks the segment again)
     uint16_t* rel;  // [nseg][REC_MAXR]: each record start of the walk, less its first
+    uint32_t* pre;  // [nseg][REC_MAXR]: the sums over the walk's records before each (pre_pack)
+    int64_t* cig_base;   // exclusive scans over the segments (walks off the chain count 0) [nseg + 1]
+    int64_t* seq_base;   //   of seqb
+    int64_t* qual_base;  //   of qual
     void* scan_ws;
     int64_t nseg;
 };
@@ -46,6 +50,20 @@ struct RecWs {
 // record starts a segment's walk records: a record takes at least 4 + 32 + 1 bytes, so at most
 // REC_SEG / 37 + 1 of them start in a segment
 constexpr int REC_MAXR = (int)(REC_SEG / 37 + 2);
+
+// The records before record r of a walk lie inside its segment (each ends where the next starts,
+// at or before r), and a record holds its 4 * n_cigar_op CIGAR bytes and l_seq quality bytes
+// (the structural test checks both fit in block_size): over them n_cigar_op sums to less than
+// REC_SEG / 4 and l_seq to less than REC_SEG, and at most REC_MAXR have an odd l_seq. Packed:
+// bits 0-10 the n_cigar_op sum, 11-23 the l_seq sum, 24-31 the odd l_seq count.
+static_assert(REC_SEG <= 8192 && REC_MAXR < 256, "the packed in-segment sums need REC_SEG <= 8 KiB");
+__host__ __device__ constexpr uint32_t pre_pack(uint32_t cig, uint32_t qual, uint32_t odd) {
+    return cig | (qual << 11) | (odd << 24);
+}
+__device__ __forceinline__ uint32_t pre_cig(uint32_t p) { return p & 0x7ffu; }
+__device__ __forceinline__ uint32_t pre_qual(uint32_t p) { return (p >> 11) & 0x1fffu; }
+// bases before the record: sum of ceil(l_seq / 2) = (sum of l_seq + odd ones) / 2
+__device__ __forceinline__ uint32_t pre_seqb(uint32_t p) { return (pre_qual(p) + (p >> 24)) >> 1; }
 
 inline int64_t rec_nseg(int64_t len) { return (len + REC_SEG - 1) / REC_SEG; }
 
@@ -66,6 +84,10 @@ inline RecWs rec_ws_layout_seg(void* ws, int64_t nseg) {
     w.base = p; p += nseg + 1;
     w.nrel = p; p += nseg;
     w.rel = (uint16_t*)p; p += (nseg * REC_MAXR + 3) / 4;
+    w.pre = (uint32_t*)p; p += (nseg * REC_MAXR + 1) / 2;
+    w.cig_base = p; p += nseg + 1;
+    w.seq_base = p; p += nseg + 1;
+    w.qual_base = p; p += nseg + 1;
     w.scan_ws = p;
     return w;
 }
@@ -73,7 +95,8 @@ inline RecWs rec_ws_layout_seg(void* ws, int64_t nseg) {
 inline RecWs rec_ws_layout(void* ws, int64_t len) { return rec_ws_layout_seg(ws, rec_nseg(len)); }
 
 inline int64_t rec_ws_bytes_seg(int64_t nseg) {
-    return 8 * (8 + 10 * nseg + 1) + 8 * ((nseg * REC_MAXR + 3) / 4) + scan_workspace(nseg) + 64;
+    return 8 * (8 + 10 * nseg + 1) + 8 * ((nseg * REC_MAXR + 3) / 4) + 8 * ((nseg * REC_MAXR + 1) / 2) +
+           8 * 3 * (nseg + 1) + scan_workspace(nseg) + 64;
 }
 
 inline int64_t rec_ws_bytes(int64_t len) { return rec_ws_bytes_seg(rec_nseg(len)); }
@@ -89,12 +112,12 @@ __device__ __forceinline__ bool record_start_at(const A& a, int64_t n, int64_t o
 }
 
 struct WalkOut {
-    int64_t e, cnt, cig, seqb, qual, flag;
+    int64_t e, cnt, cig, seqb, qual, flag, odd;
 };
 
 // records that start in [o, hi) from o, a record start
 __device__ __forceinline__ WalkOut walk(const uint8_t* b, int64_t n, int64_t o, int64_t hi, int n_ref) {
-    WalkOut w = {o, 0, 0, 0, 0, 0};
+    WalkOut w = {o, 0, 0, 0, 0, 0, 0};
     while (o < hi) {
         if (o + 4 > n) { w.flag = SEG_END_DATA; break; }
         int64_t bs = (int64_t)ld_u32(b, o);
@@ -130,24 +153,17 @@ __global__ void __launch_bounds__(REC_NT) rec_links(int64_t start, RecWs w) {
     if (lane_id() == 0 && bad) atomicAdd((unsigned long long*)&w.hdr[0], (unsigned long long)bad);
 }
 
-// Every walk but the first must be landed on exactly once; then sum the walks.
+// Every walk but the first must be landed on exactly once; walks that stop early. (The record
+// and CSR totals come from the scans over the segments.)
 __global__ void __launch_bounds__(REC_NT) rec_sum(RecWs w, int64_t* result) {
     int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
-    int64_t bad = 0, cnt = 0, cig = 0, seqb = 0, qual = 0, inv = 0, tail = -1;
+    int64_t bad = 0, inv = 0, tail = -1;
     if (k < w.nseg && w.f[k] >= 0) {
         if (k > 0 && w.pred[k] != 1) bad++;
-        cnt = w.cnt[k];
-        cig = w.cig[k];
-        seqb = w.seqb[k];
-        qual = w.qual[k];
         if (w.flag[k]) tail = w.e[k];
         if (w.flag[k] == SEG_END_INVALID) inv++;
     }
     bad = wave_sum(bad);
-    cnt = wave_sum(cnt);
-    cig = wave_sum(cig);
-    seqb = wave_sum(seqb);
-    qual = wave_sum(qual);
     inv = wave_sum(inv);
     for (int d = 32; d; d >>= 1) {
         int64_t t = __shfl_xor(tail, d, CBM_WAVE);
@@ -155,14 +171,17 @@ __global__ void __launch_bounds__(REC_NT) rec_sum(RecWs w, int64_t* result) {
     }
     if (lane_id() == 0) {
         if (bad) atomicAdd((unsigned long long*)&w.hdr[0], (unsigned long long)bad);
-        if (cnt) atomicAdd((unsigned long long*)&result[CBM_REC_N], (unsigned long long)cnt);
-        if (cig) atomicAdd((unsigned long long*)&result[CBM_REC_CIGAR], (unsigned long long)cig);
-        if (seqb) atomicAdd((unsigned long long*)&result[CBM_REC_SEQ], (unsigned long long)seqb);
-        if (qual) atomicAdd((unsigned long long*)&result[CBM_REC_QUAL], (unsigned long long)qual);
         if (inv) atomicAdd((unsigned long long*)&result[CBM_REC_INVALID], (unsigned long long)inv);
         if (tail >= 0) atomicMax((long long*)&result[CBM_REC_TAIL], (long long)tail);
     }
 }
+
+// a segment's value, 0 for a walk off the chain (f < 0)
+struct SegVal {
+    const int64_t* f;
+    const int64_t* v;
+    __device__ int64_t operator()(int64_t k) const { return f[k] >= 0 ? v[k] : 0; }
+};
 
 __global__ void rec_result_init(int64_t* result, int64_t start, RecWs w) {
     for (int i = threadIdx.x; i < CBM_REC_RESULT_LEN; i += blockDim.x) result[i] = 0;
@@ -175,28 +194,62 @@ __global__ void rec_result_init(int64_t* result, int64_t start, RecWs w) {
 
 __global__ void rec_result_links(int64_t* result, RecWs w) { result[CBM_REC_BAD_LINKS] = w.hdr[0]; }
 
-// offsets of the records of each walk, at the walk's base
+// CSR offset columns written with the record offsets (null pointers: none), [n_rec + 1] each
+struct CsrOut {
+    int64_t* cig;
+    int64_t* seq;
+    int64_t* qual;
+};
+
+// offsets of the records of each walk, at the walk's base, and with them the CSR offsets of their
+// CIGAR words, bases and qualities (the segment's scanned base plus the walk's in-segment sums)
 // a wavefront per segment: lanes copy its recorded starts, coalesced; a segment whose starts were
-// not recorded is walked by lane 0
+// not recorded is walked by lane 0. Entry n_rec of a CSR column is written with the record
+// n_rec when there is one (a caller taking fewer records than the chain holds).
 constexpr int EMIT_SEGS = REC_NT / CBM_WAVE;  // segments per workgroup
 __global__ void __launch_bounds__(REC_NT) rec_emit(const uint8_t* b, int64_t start, int64_t n, RecWs w,
-                                                   int64_t* off, int64_t n_rec) {
+                                                   int64_t* off, int64_t n_rec, CsrOut csr) {
     const int lane = (int)threadIdx.x % CBM_WAVE;
     const int64_t k = (int64_t)blockIdx.x * EMIT_SEGS + (int64_t)threadIdx.x / CBM_WAVE;
     if (k >= w.nseg) return;
     int64_t o = w.f[k];
     if (o < 0) return;
     const int64_t c = w.cnt[k], idx = w.base[k];
+    const int64_t cb = csr.cig ? w.cig_base[k] : 0, sb = csr.seq ? w.seq_base[k] : 0,
+                  qb = csr.qual ? w.qual_base[k] : 0;
     if (w.nrel[k] == c) {  // the walk recorded every start: no chain of dependent loads
         const uint16_t* rel = w.rel + k * REC_MAXR;
-        for (int64_t i = lane; i < c && idx + i < n_rec; i += CBM_WAVE) off[idx + i] = o + rel[i];
+        const uint32_t* pre = w.pre + k * REC_MAXR;
+        for (int64_t i = lane; i < c && idx + i <= n_rec; i += CBM_WAVE) {
+            if (idx + i < n_rec) off[idx + i] = o + rel[i];
+            const uint32_t p = pre[i];
+            if (csr.cig) csr.cig[idx + i] = cb + pre_cig(p);
+            if (csr.seq) csr.seq[idx + i] = sb + pre_seqb(p);
+            if (csr.qual) csr.qual[idx + i] = qb + pre_qual(p);
+        }
         return;
     }
     if (lane) return;
-    for (int64_t i = 0; i < c && idx + i < n_rec; i++) {
-        off[idx + i] = o;
+    int64_t sc = 0, ss = 0, sq = 0;
+    for (int64_t i = 0; i < c && idx + i <= n_rec; i++) {
+        if (idx + i < n_rec) off[idx + i] = o;
+        if (csr.cig) csr.cig[idx + i] = cb + sc;
+        if (csr.seq) csr.seq[idx + i] = sb + ss;
+        if (csr.qual) csr.qual[idx + i] = qb + sq;
+        const uint32_t lseq = ld_u32(b, o + 20);
+        sc += ld_u16(b, o + 16);
+        ss += (lseq + 1) / 2;
+        sq += lseq;
         o += 4 + (int64_t)ld_u32(b, o);
     }
+}
+
+// entry n_rec of the CSR columns when n_rec is the whole chain: the totals
+__global__ void rec_csr_totals(RecWs w, int64_t n_rec, CsrOut csr) {
+    if (w.base[w.nseg] != n_rec) return;
+    if (csr.cig) csr.cig[n_rec] = w.cig_base[w.nseg];
+    if (csr.seq) csr.seq[n_rec] = w.seq_base[w.nseg];
+    if (csr.qual) csr.qual[n_rec] = w.qual_base[w.nseg];
 }
 
 // ------------------------------------------------------------------ kernel 3 over many spans
@@ -243,28 +296,16 @@ __global__ void __launch_bounds__(REC_NT) recs_links(SpanTab t, RecWs w) {
 
 __global__ void __launch_bounds__(REC_NT) recs_sum(SpanTab t, RecWs w, int64_t* result) {
     int64_t k = (int64_t)blockIdx.x * REC_NT + threadIdx.x;
-    int64_t bad = 0, cnt = 0, cig = 0, seqb = 0, qual = 0, inv = 0;
+    int64_t bad = 0, inv = 0;
     if (k < w.nseg && w.f[k] >= 0) {
         int64_t s = span_of(t, k);
         if (k > t.segbase[s] && w.pred[k] != 1) bad++;
-        cnt = w.cnt[k];
-        cig = w.cig[k];
-        seqb = w.seqb[k];
-        qual = w.qual[k];
         if (w.flag[k] == SEG_END_INVALID || (w.flag[k] && w.e[k] != t.end[s])) inv++;
     }
     bad = wave_sum(bad);
-    cnt = wave_sum(cnt);
-    cig = wave_sum(cig);
-    seqb = wave_sum(seqb);
-    qual = wave_sum(qual);
     inv = wave_sum(inv);
     if (lane_id() == 0) {
         if (bad) atomicAdd((unsigned long long*)&w.hdr[0], (unsigned long long)bad);
-        if (cnt) atomicAdd((unsigned long long*)&result[CBM_REC_N], (unsigned long long)cnt);
-        if (cig) atomicAdd((unsigned long long*)&result[CBM_REC_CIGAR], (unsigned long long)cig);
-        if (seqb) atomicAdd((unsigned long long*)&result[CBM_REC_SEQ], (unsigned long long)seqb);
-        if (qual) atomicAdd((unsigned long long*)&result[CBM_REC_QUAL], (unsigned long long)qual);
         if (inv) atomicAdd((unsigned long long*)&result[CBM_REC_INVALID], (unsigned long long)inv);
     }
 }
@@ -316,8 +357,9 @@ struct StagedBytes {
 // through block_size is followed 64 records at a time (the same by every lane; lane j keeps the
 // j-th offset), then each lane tests its record; the first that fails ends the walk there.
 template <class A>
-__device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane, uint16_t* rel) {
-    WalkOut w = {o, 0, 0, 0, 0, 0};
+__device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane, uint16_t* rel,
+                            uint32_t* pre) {
+    WalkOut w = {o, 0, 0, 0, 0, 0, 0};
     const int64_t o0 = o;
     while (o < hi && !w.flag) {
         int64_t mine = -1;
@@ -333,18 +375,25 @@ __device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n
         const bool bad = mine >= 0 && !is_record_at(a, n, mine, n_ref);
         const uint64_t badm = __ballot(bad);
         const int good = badm ? __ffsll((unsigned long long)badm) - 1 : nrec;
-        int64_t cig = 0, seqb = 0, qual = 0;
-        if (lane < good && w.cnt + lane < REC_MAXR) rel[w.cnt + lane] = (uint16_t)(mine - o0);
+        uint32_t ncig = 0, lseq = 0;
         if (lane < good) {
-            const uint32_t lseq = a.u32(mine + 20);
-            cig = a.u16(mine + 16);
-            seqb = (lseq + 1) / 2;
-            qual = lseq;
+            lseq = a.u32(mine + 20);
+            ncig = a.u16(mine + 16);
+        }
+        // the sums over the walk's records before each (see pre_pack): n_cigar_op and the odd
+        // l_seq count in one word, whose fields hold every exclusive prefix without carries
+        const uint32_t co = ncig + ((lseq & 1u) << 16);
+        const uint32_t xco = wave_inclusive_sum(co) - co, xq = wave_inclusive_sum(lseq) - lseq;
+        if (lane < good && w.cnt + lane < REC_MAXR) {
+            rel[w.cnt + lane] = (uint16_t)(mine - o0);
+            pre[w.cnt + lane] = pre_pack((uint32_t)w.cig + (xco & 0xffffu), (uint32_t)w.qual + xq,
+                                         (uint32_t)w.odd + (xco >> 16));
         }
         w.cnt += good;
-        w.cig += wave_sum(cig);
-        w.seqb += wave_sum(seqb);
-        w.qual += wave_sum(qual);
+        w.cig += wave_sum((int64_t)ncig);
+        w.seqb += wave_sum((int64_t)((lseq + 1) / 2));
+        w.qual += wave_sum((int64_t)lseq);
+        w.odd += wave_sum((int64_t)(lseq & 1u));
         if (badm) {
             o = __shfl(mine, good, CBM_WAVE);
             w.flag = SEG_END_INVALID;
@@ -408,8 +457,8 @@ __global__ void __launch_bounds__(RW_WAVES * CBM_WAVE)
             }
         }
     }
-    WalkOut r = {-1, 0, 0, 0, 0, 0};
-    if (f >= 0) r = walk_wave(a, n, f, hi, n_ref, lane, w.rel + k * REC_MAXR);
+    WalkOut r = {-1, 0, 0, 0, 0, 0, 0};
+    if (f >= 0) r = walk_wave(a, n, f, hi, n_ref, lane, w.rel + k * REC_MAXR, w.pre + k * REC_MAXR);
     if (lane == 0) {
         w.f[k] = f;
         w.nrel[k] = r.cnt <= REC_MAXR ? r.cnt : -1;

@@ -271,11 +271,13 @@ def decode_staged(staged: Staged, tags=("CB", "UB"), ws: _decode.Workspace | Non
     bounds = h[9 : 10 + n].numpy().copy()
     # ---- offsets and kernel 4 over all records
     off = torch.empty(max(n_rec, 1), dtype=_I64, device=dev)
-    check(L.cbm_record_offsets_spans(ptr(data), n_seg, ptr(rws), rws.numel(), ptr(off), n_rec, st),
-          "cbm_record_offsets_spans")
+    csr = {k: torch.empty(n_rec + 1, dtype=_I64, device=dev) for k in _decode.CSR_OFFSETS}
+    check(L.cbm_record_offsets_spans_csr(ptr(data), n_seg, ptr(rws), rws.numel(), ptr(off),
+                                         *(ptr(csr[k]) for k in _decode.CSR_OFFSETS), n_rec, st),
+          "cbm_record_offsets_spans_csr")
     off = off[:n_rec]
     sizes = SimpleNamespace(n_cigar=int(h[_lib.REC_CIGAR]), n_seq=int(h[_lib.REC_SEQ]), n_qual=int(h[_lib.REC_QUAL]))
-    out = _decode.scatter(data, off, sizes, tags=tags, ws=ws)
+    out = _decode.scatter(data, off, sizes, tags=tags, ws=ws, csr=csr)
     fixed, tag_cols = out["fixed"], out["tags"]
     res = []
     for r, (src, sp) in enumerate(items):
@@ -309,8 +311,8 @@ def decode_plain(src: BamSource, sp: Span, tags=("CB", "UB"), ws: _decode.Worksp
     chain = _decode.record_chain(inf.data, start, end, src.n_ref, ws, inf=inf)
     if chain.tail != end or chain.n_rec != sp.records:
         raise ValueError(_decode.MSG_NO_BLOCK)
-    off = _decode.record_offsets(inf.data, start, end, chain)
-    return RecordBatch.decode(inf.data, off, src.header, chain, tags)
+    off, csr = _decode.record_offsets(inf.data, start, end, chain, csr=True)
+    return RecordBatch.decode(inf.data, off, src.header, chain, tags, csr=csr)
 
 
 # ------------------------------------------------------------------ the request loop

@@ -218,6 +218,17 @@ int cbm_record_offsets(const uint8_t* data_d, int64_t start, int64_t n, const vo
                        int64_t* offsets_d, int64_t n_rec, void* stream);
 
 /*
+ * cbm_record_offsets, and with the offsets the CSR offset columns of the records' CIGAR words,
+ * packed bases and qualities ([n_rec + 1] each, as cbm_scatter writes them; a NULL column is
+ * skipped), from the sums the chain's walks kept: no pass over the records. For
+ * cbm_scatter_csr. The reference computes them from the decoded batch (cumsum of n_cigar_op
+ * and l_seq, src/cubam/bam.py:275-288).
+ */
+int cbm_record_offsets_csr(const uint8_t* data_d, int64_t start, int64_t n, const void* ws_d, int64_t ws_bytes,
+                           int64_t* offsets_d, int64_t* cigar_off_d, int64_t* seq_off_d, int64_t* qual_off_d,
+                           int64_t n_rec, void* stream);
+
+/*
  * Record chains of many spans of one buffer in one launch sequence: the records of
  * span s are data_d[span_start_d[s] : span_end_d[s]), both ends record boundaries
  * (a batch of region requests, each inflated from a block where a record starts
@@ -242,6 +253,10 @@ int cbm_record_chain_spans_repair(const uint8_t* data_d, const int64_t* span_sta
 /* Offsets (from data start) of the n_rec records of all spans, span by span. */
 int cbm_record_offsets_spans(const uint8_t* data_d, int64_t n_seg, const void* ws_d, int64_t ws_bytes,
                              int64_t* offsets_d, int64_t n_rec, void* stream);
+/* cbm_record_offsets_spans with the CSR offset columns, as cbm_record_offsets_csr. */
+int cbm_record_offsets_spans_csr(const uint8_t* data_d, int64_t n_seg, const void* ws_d, int64_t ws_bytes,
+                                 int64_t* offsets_d, int64_t* cigar_off_d, int64_t* seq_off_d, int64_t* qual_off_d,
+                                 int64_t n_rec, void* stream);
 
 /* ------------------------------------------------------------------ column scatter (kernel 4) */
 
@@ -279,6 +294,14 @@ int64_t cbm_scatter_workspace(int64_t n_rec);
  */
 int cbm_scatter(const uint8_t* data_d, const int64_t* offsets_d, int64_t n_rec, const cbm_columns* cols, void* ws_d,
                 int64_t ws_bytes, void* stream);
+
+/*
+ * cbm_scatter for records whose CSR offset columns are already written (cols->cigar_off,
+ * seq_off and qual_off are inputs here, from cbm_record_offsets_csr for the same records):
+ * needs no workspace. Replaces the same RecordBatch.decode (src/cubam/bam.py:268-288).
+ */
+int cbm_scatter_csr(const uint8_t* data_d, const int64_t* offsets_d, int64_t n_rec, const cbm_columns* cols,
+                    void* stream);
 
 /* ------------------------------------------------------------------ on-demand record kernels */
 /* One thread per record, reading records in place at data_d + off_d[i] (batch offsets). */
