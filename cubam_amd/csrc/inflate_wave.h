@@ -256,10 +256,10 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
         if (d & 0x8000) d = s.dist[((d >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(x, DBITS, d & 15)];
         const int dl = (int)(d & 15), de = (int)((d >> 4) & 15), ds = (int)((d >> 8) & 31);
         const bool bad = !dl || ds >= 30;
-        int db, de2;
-        dist_code(ds, db, de2);
         r.kind = bad ? (int)KIND_BAD : (int)KIND_LEN;
         r.bits = bad ? u + (dl ? dl : 1) : u + dl + de;
+        int db, de2;
+        dist_code(ds, db, de2);
         r.dist = db + (int)__builtin_amdgcn_ubfe(x, dl, de);
     }
     return r;
@@ -1566,40 +1566,31 @@ __global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve(uint8_t* dst, 
                     stuck = true;                                  // leave the block to the one-phase kernel
                     break;
                 }
+                // the ready matches, one at a time, each copied by the whole wavefront (a byte a
+                // lane): their destinations are disjoint and pending, their sources final, so they
+                // go in any order. Byte i of a match reads src + i, or src + i mod dist when it
+                // overlaps its own output (a repeat of the dist final bytes before dest).
 #pragma unroll
                 for (int u = 0; u < MPL; u++) {
-                    if (!(rdy & (1u << u))) continue;
-                    // eight bytes a pass, loads before stores: every byte read is final (an
-                    // overlapping copy repeats [src, dest), r = i mod dist)
-                    const int src = dest[u] - dist[u], dl = dist[u], ln = len[u];
-                    uint8_t* d = sp + shift + dest[u] - lo;
-                    int r = 0;
-                    if (src >= lo) {  // the source is in the span: LDS only
-                        const uint8_t* sl = sp + shift + src - lo;
-                        for (int i0 = 0; i0 < ln; i0 += 8) {
-                            uint8_t t[8];
-#pragma unroll
-                            for (int j = 0; j < 8; j++) {
-                                t[j] = i0 + j < ln ? sl[dl >= ln ? i0 + j : r] : 0;
-                                r = r + 1 == dl ? 0 : r + 1;
+                    for (uint64_t rm = __ballot((rdy >> u) & 1u); rm; rm &= rm - 1) {
+                        const int l = __builtin_ctzll(rm);
+                        const int md = __builtin_amdgcn_readlane(dest[u], l);
+                        const int dl = __builtin_amdgcn_readlane(dist[u], l);
+                        const int ln = __builtin_amdgcn_readlane(len[u], l);
+                        const int src = md - dl;
+                        uint8_t* d = sp + shift + md - lo;
+                        if (src >= lo) {  // the source is in the span: LDS only
+                            const uint8_t* sl = sp + shift + src - lo;
+                            if (dl >= ln) {
+                                for (int i = lane; i < ln; i += CBM_WAVE) d[i] = sl[i];
+                            } else {
+                                for (int i = lane; i < ln; i += CBM_WAVE) d[i] = sl[i % dl];
                             }
-#pragma unroll
-                            for (int j = 0; j < 8; j++)
-                                if (i0 + j < ln) d[i0 + j] = t[j];
-                        }
-                    } else {  // partly before the span (in dst), partly in it
-                        for (int i0 = 0; i0 < ln; i0 += 8) {
-                            uint8_t t[8];
-#pragma unroll
-                            for (int j = 0; j < 8; j++) {
-                                const int i = i0 + j;
-                                const int q = src + (dl >= ln ? i : r);
-                                t[j] = i < ln ? (q < lo ? out[q] : sp[shift + q - lo]) : 0;
-                                r = r + 1 == dl ? 0 : r + 1;
+                        } else {  // partly before the span (in dst), partly in it
+                            for (int i = lane; i < ln; i += CBM_WAVE) {
+                                const int q = src + (dl >= ln ? i : i % dl);
+                                d[i] = q < lo ? out[q] : sp[shift + q - lo];
                             }
-#pragma unroll
-                            for (int j = 0; j < 8; j++)
-                                if (i0 + j < ln) d[i0 + j] = t[j];
                         }
                     }
                 }
