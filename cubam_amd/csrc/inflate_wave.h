@@ -148,31 +148,6 @@ __device__ __forceinline__ uint32_t peek32(const S& s, int p, int wbase) {
     return __builtin_amdgcn_alignbit(s.in[w + 1], s.in[w], p & 31);
 }
 
-// Wavefront shifts and scans on the data-parallel primitives (DPP: register moves between
-// lanes, no LDS): lane i gets x of lane i - 1 (lane 0: fill)
-__device__ __forceinline__ int wave_up1(int x, int fill) {
-    return __builtin_amdgcn_update_dpp(fill, x, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
-}
-// sum (max) of x over lanes 0..i: steps of 1, 2, 4, 8 lanes within rows of 16 (a lane with no
-// source adds 0), then row 15 into rows 1 and 3, then lane 31 into rows 2 and 3
-__device__ __forceinline__ int wave_scan_sum(int v) {
-    v += __builtin_amdgcn_update_dpp(0, v, 0x111 /* row_shr:1 */, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x112 /* row_shr:2 */, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x114 /* row_shr:4 */, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x118 /* row_shr:8 */, 0xf, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x142 /* row_bcast:15 */, 0xa, 0xf, true);
-    v += __builtin_amdgcn_update_dpp(0, v, 0x143 /* row_bcast:31 */, 0xc, 0xf, true);
-    return v;
-}
-__device__ __forceinline__ int wave_scan_max(int v) {  // v >= 0
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, true));
-    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, true));
-    return v;
-}
 
 // one code-length symbol at bit p, looked up in the code-length table (7-bit, no subtables) held
 // in the distance table's space: its bits (code and extra bits; 0 for an invalid code), the

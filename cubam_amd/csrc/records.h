@@ -354,19 +354,29 @@ struct StagedBytes {
 };
 
 // walk() with the wavefront: records that start in [o, hi) from o, a record start. The chain
-// through block_size is followed 64 records at a time (the same by every lane; lane j keeps the
-// j-th offset), then each lane tests its record; the first that fails ends the walk there.
+// through block_size is followed 64 records at a time by the wavefront as one (scalar registers;
+// lane j keeps the j-th offset), then each lane tests its record; the first that fails ends the
+// walk there. The sums over the records (n_cigar_op, l_seq, odd l_seq) are DPP scans and ballots.
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 template <class A>
 __device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane, uint16_t* rel,
                             uint32_t* pre) {
     WalkOut w = {o, 0, 0, 0, 0, 0, 0};
+    o = uniform64(o);
+    n = uniform64(n);
+    hi = uniform64(hi);
     const int64_t o0 = o;
     while (o < hi && !w.flag) {
         int64_t mine = -1;
         int nrec = 0;
         while (nrec < CBM_WAVE && o < hi) {
             if (o + 4 > n) { w.flag = SEG_END_DATA; break; }
-            const int64_t bs = (int64_t)a.u32(o);
+            const int64_t bs = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)a.u32(o));
             if (o + 4 + bs > n) { w.flag = SEG_END_DATA; break; }
             if (lane == nrec) mine = o;
             nrec++;
@@ -380,22 +390,27 @@ __device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n
             lseq = a.u32(mine + 20);
             ncig = a.u16(mine + 16);
         }
-        // the sums over the walk's records before each (see pre_pack): n_cigar_op and the odd
-        // l_seq count in one word, whose fields hold every exclusive prefix without carries
-        const uint32_t co = ncig + ((lseq & 1u) << 16);
-        const uint32_t xco = wave_inclusive_sum(co) - co, xq = wave_inclusive_sum(lseq) - lseq;
+        // sums over the walk's records before each (pre_pack): n_cigar_op and l_seq by inclusive
+        // scans (a record's l_seq is at most 2^22, so 64 of them fit 32 bits), odd l_seq by ballot
+        const uint32_t ic = (uint32_t)wave_scan_sum((int)ncig), iq = (uint32_t)wave_scan_sum((int)lseq);
+        const uint64_t oddm = __ballot(lseq & 1u);
+        const uint32_t xodd = __builtin_amdgcn_mbcnt_hi((uint32_t)(oddm >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)oddm, 0u));
         if (lane < good && w.cnt + lane < REC_MAXR) {
             rel[w.cnt + lane] = (uint16_t)(mine - o0);
-            pre[w.cnt + lane] = pre_pack((uint32_t)w.cig + (xco & 0xffffu), (uint32_t)w.qual + xq,
-                                         (uint32_t)w.odd + (xco >> 16));
+            pre[w.cnt + lane] = pre_pack((uint32_t)w.cig + ic - ncig, (uint32_t)w.qual + iq - lseq,
+                                         (uint32_t)w.odd + xodd);
         }
+        const int64_t tc = (uint32_t)__builtin_amdgcn_readlane((int)ic, CBM_WAVE - 1);
+        const int64_t tq = (uint32_t)__builtin_amdgcn_readlane((int)iq, CBM_WAVE - 1);
+        const int64_t todd = __popcll(oddm);
         w.cnt += good;
-        w.cig += wave_sum((int64_t)ncig);
-        w.seqb += wave_sum((int64_t)((lseq + 1) / 2));
-        w.qual += wave_sum((int64_t)lseq);
-        w.odd += wave_sum((int64_t)(lseq & 1u));
+        w.cig += tc;
+        w.qual += tq;
+        w.seqb += (tq + todd) / 2;  // sum of ceil(l_seq / 2)
+        w.odd += todd;
         if (badm) {
-            o = __shfl(mine, good, CBM_WAVE);
+            o = uniform64(__shfl(mine, good, CBM_WAVE));
             w.flag = SEG_END_INVALID;
         }
     }
