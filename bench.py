@@ -162,7 +162,7 @@ def batched_bench(n_blocks: int, steps: int, seed: int = 2):
     blocks = _decode.scan_blocks(buf, C, True, ws)
     assert blocks.n_blocks == n_blocks
     best, table = tune.autotune(blocks, sample=50_000, log=log, ws=ws)
-    default_ms = tune.time_inflate(blocks, (0, 0, 0, 0), ws=ws)
+    default_ms = tune.time_inflate(blocks, None, ws=ws)
     g = tune.GraphInflate(blocks, best)
     g.run()
     torch.cuda.synchronize()
@@ -319,7 +319,8 @@ def write_bench(buf, n, start, n_ref, ws, header, reps: int = 3):
 SERVE_FILES = ((0, 30), (0, 31), (0, 32), (0, 33), (1, 34), (1, 35), (1, 36), (1, 37))
 
 
-def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, budget_gb: float = 64.0):
+def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, budget_gb: float = 64.0,
+                codec: str = "zlib"):
     """Served vs plain region decode over 8 files held in pinned host memory.
 
     Returns the JSON object of the serving comparison: closed-loop (every request
@@ -335,13 +336,15 @@ def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, bud
     srcs = []
     for kind, seed in SERVE_FILES:
         n = gex_records if kind == 0 else atac_records
-        b = synth.make_bam(kind, seed, n, n_total=n)
+        b = synth.make_bam(kind, seed, n, n_total=n, codec=codec)
         srcs.append(BamSource(b.bgzf, name=f"{'GEX' if kind == 0 else 'ATAC'}-{seed}"))
         del b
+        log(f"serving file {len(srcs)} of {len(SERVE_FILES)} ready ({time.time() - t:.0f} s)")
+    gen_s = time.time() - t
     c_total = sum(s.size for s in srcs)
     u_total = sum(s.index.stream_bytes for s in srcs)
     log(f"serving: 8 files, {c_total / 1e9:.2f} GB compressed, {u_total / 1e9:.2f} GB inflated,"
-        f" generated and indexed in {time.time() - t:.1f} s")
+        f" generated ({codec} level 6) and indexed in {gen_s:.1f} s")
     budget = int(budget_gb * 1e9)
 
     def run(mode, trace):
@@ -390,6 +393,8 @@ def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, bud
                      f" {atac_records}); region requests, compressed span log-uniform 64 KiB-256 MiB"),
         "compressed_bytes": c_total,
         "inflated_bytes": u_total,
+        "compressor": f"{codec} level 6",
+        "generate_and_index_s": round(gen_s, 1),
         "budget_GB": budget_gb,
         "requests_per_run": n_requests,
         "closed_loop": {"plain": plain, "served": served,
@@ -414,6 +419,8 @@ def main():
     ap.add_argument("--batched-blocks", type=int, default=500_000)
     ap.add_argument("--serve-gex-records", type=int, default=2_000_000)
     ap.add_argument("--serve-atac-records", type=int, default=4_000_000)
+    ap.add_argument("--serve-codec", choices=("zlib", "libdeflate"), default="zlib",
+                    help="compressor of the serving files (libdeflate: same level, about 3x faster to generate)")
     ap.add_argument("--serve-requests", type=int, default=160)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -516,7 +523,7 @@ def main():
         res["batched"] = batched_bench(args.batched_blocks, max(3, min(args.steps, 10)))
     if not args.no_serve:
         res["serving"] = serve_bench(args.serve_gex_records, args.serve_atac_records, args.serve_requests,
-                                     rates=(0.5, 1.0, 2.0))
+                                     rates=(0.5, 1.0, 2.0), codec=args.serve_codec)
     if not args.no_cpu:
         v, cores, sample = cpu_baseline(chunk, hdr.size, n_ref)
         v1, _, sample1 = cpu_baseline(chunk, hdr.size, n_ref, cores=1, sample_mib=32)
@@ -544,11 +551,17 @@ def profiled_traffic(shape_label: str):
 
 def _kernel_name() -> str:
     from cubam_amd import _decode
-    from cubam_amd._lib import INFLATE_SHAPES
+    from cubam_amd._lib import INFLATE_DEFAULT, INFLATE_SHAPES, INFLATE_TWO_PHASE
 
-    shape = INFLATE_SHAPES[_decode.get_inflate_tune()[0]]
+    t = _decode.get_inflate_tune()
+    v = t[0] if any(t) else INFLATE_DEFAULT
+    shape = INFLATE_SHAPES[v]
     if shape is None:
         return "cbm_inflate: inflate_lanes + crc_blocks"
+    if v in INFLATE_TWO_PHASE:
+        mpl, wpg = t[1] or 1, t[2] or 4
+        return (f"cbm_inflate (two-phase): inflate_wave<64, {shape[1]}, true> + "
+                f"inflate_resolve<{mpl}, {2048 * mpl}, {wpg}> (CRC32 fused in the resolve)")
     return f"cbm_inflate: inflate_wave<{shape[0]}, {shape[1]}> (CRC32 fused)"
 
 

@@ -33,6 +33,8 @@ INFLATE_SHAPES = {0: (512, 128), 1: None, 2: (128, 256), 3: (512, 64), 4: (64, 5
                   13: (64, 256)}
 #: the two-phase variants (decode to literals and match lists, then resolve)
 INFLATE_TWO_PHASE = (INFLATE_TOK64_S512, INFLATE_TOK64_S1024, INFLATE_TOK64_S256)
+#: the variant a null tune selects (CBM_INFLATE_DEFAULT)
+INFLATE_DEFAULT = INFLATE_TOK64_S512
 
 
 class InflateTune(ctypes.Structure):

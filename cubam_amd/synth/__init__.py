@@ -57,8 +57,10 @@ def lib():
         L.cbs_records.argtypes = [vp, vp, i64, i64, ctypes.POINTER(i64)]
         L.cbs_contig_length.restype = i64
         L.cbs_contig_length.argtypes = [vp, ctypes.c_int]
-        L.cbs_bgzf_blocks.restype = ctypes.c_int
-        L.cbs_bgzf_blocks.argtypes = [vp, vp, vp, i64, vp, vp, vp, i64, vp, ctypes.c_int]
+        L.cbs_bgzf_blocks_codec.restype = ctypes.c_int
+        L.cbs_bgzf_blocks_codec.argtypes = [vp, vp, vp, i64, vp, vp, vp, i64, vp, ctypes.c_int, ctypes.c_int]
+        L.cbs_have_libdeflate.restype = ctypes.c_int
+        L.cbs_have_libdeflate.argtypes = []
         _lib = L
     return _lib
 
@@ -69,6 +71,16 @@ def _ptr(a: np.ndarray | None):
 
 def default_threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))
+
+
+#: compressors of ``bgzf_blocks``: zlib, or libdeflate at the same level (about three times faster
+#: on these records, compressed sizes within 1 % of zlib's; levels 1-9 without fixed codes only)
+CODECS = {"zlib": 0, "libdeflate": 1}
+
+
+def have_libdeflate() -> bool:
+    """Whether ``codec="libdeflate"`` can be used (its shared library is present)."""
+    return bool(lib().cbs_have_libdeflate())
 
 
 class RecordStream:
@@ -98,9 +110,9 @@ class RecordStream:
         return [int(lib().cbs_contig_length(self._h, c)) for c in range(25)]
 
     def close(self) -> None:
-        if self._h:
-            lib().cbs_close(self._h)
-            self._h = None
+        if self._h and _lib is not None:
+            _lib.cbs_close(self._h)
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -113,6 +125,7 @@ def bgzf_blocks(
     level: np.ndarray | int = 6,
     fixed: np.ndarray | None = None,
     n_threads: int | None = None,
+    codec: str = "zlib",
 ) -> tuple[np.ndarray, np.ndarray]:
     """BGZF blocks of the payloads ``data[off[i] : off[i] + size[i]]``; (bytes, block sizes)."""
     off = np.ascontiguousarray(off, np.int64)
@@ -132,13 +145,15 @@ def bgzf_blocks(
     parts = []
     for b0 in range(0, n, batch):
         b1 = min(n, b0 + batch)
-        rc = lib().cbs_bgzf_blocks(
+        rc = lib().cbs_bgzf_blocks_codec(
             _ptr(data), _ptr(off[b0:b1]), _ptr(size[b0:b1]), b1 - b0, _ptr(lv[b0:b1]),
             None if fx is None else _ptr(fx[b0:b1]), _ptr(tmp), stride, _ptr(bsize[b0:b1]),
-            n_threads or default_threads(),
+            n_threads or default_threads(), CODECS[codec],
         )
+        if rc == -3:
+            raise RuntimeError("libdeflate is not available")
         if rc != 0:
-            raise RuntimeError("zlib failed")
+            raise RuntimeError(f"{codec} failed")
         bs = bsize[b0:b1]
         part = np.empty(int(bs.sum(dtype=np.int64)), np.uint8)
         pos = 0
@@ -184,10 +199,12 @@ def make_bam(
     n_threads: int | None = None,
     eof: bool = True,
     piece_bytes: int = 256 << 20,
+    codec: str = "zlib",
 ) -> SynthBam:
     """The first ``n_records`` records of a synthetic file of ``n_total`` records, as BGZF.
 
-    ``kind`` 0 is GEX-like single-end 90 bp, 1 ATAC-like paired-end 2 x 50 bp.
+    ``kind`` 0 is GEX-like single-end 90 bp, 1 ATAC-like paired-end 2 x 50 bp. ``codec`` names the
+    compressor (:data:`CODECS`).
     """
     rs = RecordStream(kind, seed, n_total or n_records)
     try:
@@ -195,7 +212,7 @@ def make_bam(
         lens = rs.contig_lengths()
         hoff, hsize = _cut(len(header), block)
         parts, sizes = [], []
-        hb, hs = bgzf_blocks(np.frombuffer(header, np.uint8), hoff, hsize, level, n_threads=n_threads)
+        hb, hs = bgzf_blocks(np.frombuffer(header, np.uint8), hoff, hsize, level, n_threads=n_threads, codec=codec)
         parts.append(hb)
         sizes.append(hs)
         done, raw = 0, 0
@@ -210,13 +227,13 @@ def make_bam(
             last = done >= n_records
             full = buf.size if last else buf.size // block * block
             off, size = _cut(full, block)
-            b, s = bgzf_blocks(buf, off, size, level, n_threads=n_threads)
+            b, s = bgzf_blocks(buf, off, size, level, n_threads=n_threads, codec=codec)
             parts.append(b)
             sizes.append(s)
             carry = buf[full:].copy()
         if carry.size:
             off, size = _cut(carry.size, block)
-            b, s = bgzf_blocks(carry, off, size, level, n_threads=n_threads)
+            b, s = bgzf_blocks(carry, off, size, level, n_threads=n_threads, codec=codec)
             parts.append(b)
             sizes.append(s)
         if eof:

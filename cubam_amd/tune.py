@@ -106,8 +106,9 @@ def record(n_blocks: int, inflated: int, tune, ms: float, table: dict | None = N
 
 
 def time_inflate(blocks: _decode.Blocks, tune, reps: int = 2, ws: _decode.Workspace | None = None) -> float:
-    """Milliseconds of one inflate launch over ``blocks`` with shape ``tune`` (best of reps)."""
-    t = _lib.InflateTune(*tune)
+    """Milliseconds of one inflate launch over ``blocks`` with shape ``tune`` (best of reps); None
+    times the library's default shape."""
+    t = _lib.InflateTune(*tune) if tune is not None else None
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     best = float("inf")
     for _ in range(reps):

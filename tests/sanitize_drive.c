@@ -14,8 +14,10 @@ void* cbs_open(int kind, uint64_t seed, int64_t n_total);
 void cbs_close(void* h);
 int64_t cbs_header(void* h, uint8_t* out, int64_t cap);
 int64_t cbs_records(void* h, uint8_t* out, int64_t cap, int64_t max_rec, int64_t* n_rec);
-int cbs_bgzf_blocks(const uint8_t* src, const int64_t* off, const int32_t* len, int64_t n, const int8_t* level,
-                    const uint8_t* fixed, uint8_t* dst, int64_t stride, int32_t* bsize, int n_threads);
+int cbs_bgzf_blocks_codec(const uint8_t* src, const int64_t* off, const int32_t* len, int64_t n,
+                          const int8_t* level, const uint8_t* fixed, uint8_t* dst, int64_t stride, int32_t* bsize,
+                          int n_threads, int codec);
+int cbs_have_libdeflate(void);
 /* cpu_decode.c */
 void orc_bgzf_scan(const uint8_t* b, int64_t n, int eof, int64_t* blk_off, int32_t* bsize, int32_t* isize,
                    uint32_t* crc, int64_t cap, int64_t* out);
@@ -55,8 +57,9 @@ static uint64_t next_rand(void) {
     return rng;
 }
 
-/* the BGZF file of one synthetic recipe: header blocks, record blocks (mixed levels), EOF */
-static uint8_t* make_file(int kind, int64_t n_rec, int64_t* size, int64_t* header_bytes, int n_threads) {
+/* the BGZF file of one synthetic recipe: header blocks, record blocks (mixed levels), EOF; codec 0
+   zlib, 1 libdeflate */
+static uint8_t* make_file(int kind, int64_t n_rec, int64_t* size, int64_t* header_bytes, int n_threads, int codec) {
     void* g = cbs_open(kind, 7 + kind, n_rec);
     CHECK(g);
     uint8_t* raw = xmalloc(1 << 24);
@@ -79,7 +82,7 @@ static uint8_t* make_file(int kind, int64_t n_rec, int64_t* size, int64_t* heade
     }
     uint8_t* slots = xmalloc((size_t)nb * 65536);
     int32_t* bs = xmalloc(4 * nb);
-    CHECK(cbs_bgzf_blocks(raw, off, len, nb, lvl, fx, slots, 65536, bs, n_threads) == 0);
+    CHECK(cbs_bgzf_blocks_codec(raw, off, len, nb, lvl, fx, slots, 65536, bs, n_threads, codec) == 0);
     int64_t out = 0;
     for (int64_t i = 0; i < nb; i++) out += bs[i];
     static const uint8_t eof_block[28] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0, 0, 0xff, 6, 0, 'B', 'C', 2, 0, 0x1b, 0,
@@ -166,9 +169,11 @@ static void columns(const uint8_t* d, int64_t n, int n_ref, int n_threads) {
 }
 
 int main(void) {
-    for (int kind = 0; kind < 2; kind++) {
+    for (int run = 0; run < 4; run++) {
+        const int kind = run % 2, codec = run / 2;
+        if (codec == 1 && !cbs_have_libdeflate()) continue;
         int64_t size, hdr, n;
-        uint8_t* file = make_file(kind, 20000, &size, &hdr, 4);
+        uint8_t* file = make_file(kind, 20000, &size, &hdr, 4, codec);
         uint8_t* d = inflate_all(file, size, 1, &n, 4);
         CHECK(d && n > hdr);
         columns(d + hdr, n - hdr, 25, 4);

@@ -148,3 +148,26 @@ def test_bam_header_read_from_bgzf(tmp_path):
     p.write_bytes(bf.bgzf(bf.bam_bytes(bf.random_reads(20)), 100))
     h = BamHeader.read(p)
     assert h.size == len(bf.header_bytes()) and h.names == [n for n, _ in bf.REFS]
+
+
+def test_generator_codecs_give_the_same_records():
+    """libdeflate (the faster compressor of the synthetic files) frames the same payloads as zlib:
+    every block inflates (zlib) to the same bytes, and its trailer holds their CRC32 and size."""
+    import numpy as np
+
+    from cubam_amd import synth
+
+    if not synth.have_libdeflate():
+        pytest.skip("libdeflate is absent")
+    a = synth.make_bam(1, 5, 20_000, codec="zlib")
+    b = synth.make_bam(1, 5, 20_000, codec="libdeflate")
+    assert gzip.decompress(a.bgzf.tobytes()) == gzip.decompress(b.bgzf.tobytes())
+    assert np.array_equal(a.block_sizes > 0, b.block_sizes > 0) and a.block_sizes.size == b.block_sizes.size
+    data, o = b.bgzf.tobytes(), 0
+    for bs in b.block_sizes:
+        blk = data[o : o + int(bs)]
+        raw = zlib.decompress(blk[18:-8], -15)
+        assert int.from_bytes(blk[-8:-4], "little") == zlib.crc32(raw)
+        assert int.from_bytes(blk[-4:], "little") == len(raw) <= synth.BLOCK_DATA
+        o += int(bs)
+    assert o == len(data)
