@@ -242,6 +242,20 @@ static int launch_resolve(uint8_t* dst, const int64_t* out_off, const int32_t* i
     return check_launch("inflate_resolve");
 }
 
+// the resolve phase with a history ring: RING bytes of LDS a wavefront, WPG blocks a workgroup
+template <int RING, int WPG>
+static int launch_resolve_ring(uint8_t* dst, const int64_t* out_off, const int32_t* isize, const uint32_t* crc_expect,
+                               uint32_t* crc, int32_t* status, int64_t* result, const uint64_t* pool,
+                               const int32_t* mcount, int64_t n_blocks, hipStream_t st) {
+    using K = RingSmem<2048, WPG, RING>;
+    const int64_t per_cu = (160 * 1024) / (int64_t)((sizeof(K) + 511) / 512 * 512);
+    int64_t g = (n_blocks + WPG - 1) / WPG;
+    if (g > (int64_t)device_cus() * per_cu) g = (int64_t)device_cus() * per_cu;
+    hipLaunchKernelGGL((inflate_resolve_ring<2048, WPG, RING>), dim3((unsigned)g), dim3(WPG * CBM_WAVE), 0, st, dst,
+                       out_off, isize, crc_expect, crc, status, result, pool, mcount, n_blocks);
+    return check_launch("inflate_resolve_ring");
+}
+
 template <int SEG>
 static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                       const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
@@ -268,8 +282,22 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
         RES_CASE(1, 8)
         RES_CASE(2, 4)
         RES_CASE(2, 2)
-        RES_CASE(4, 4)
 #undef RES_CASE
+#define RING_CASE(KIB_, WPG_)                                                                                   \
+    case KIB_ * 100 + WPG_:                                                                                     \
+        rc = launch_resolve_ring<KIB_ * 1024, WPG_>(dst, out_off, isize, crc_expect, crc, status, result, pool, \
+                                                    mcount, n_blocks, st);                                      \
+        break;
+        RING_CASE(4, 1)
+        RING_CASE(4, 2)
+        RING_CASE(4, 4)
+        RING_CASE(8, 1)
+        RING_CASE(8, 2)
+        RING_CASE(8, 4)
+        RING_CASE(16, 1)
+        RING_CASE(16, 2)
+        RING_CASE(32, 1)
+#undef RING_CASE
         default:
             set_error("cbm_inflate: two-phase resolve shape (blocks_per_wave, waves_per_group) not built");
             return CBM_ERR_ARG;

@@ -1607,6 +1607,294 @@ __global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve(uint8_t* dst, 
     }
 }
 
+// ------------------------------------------------------------------ resolve phase with a history ring
+// The block's output passes once through a ring of RING bytes of LDS (a wavefront's), in stretches
+// of RES_FL bytes aligned to 16 bytes of dst (u = p + a0, a0 = the block's misalignment): each
+// stretch is staged from dst once (its literals are in place), receives the matches whose output
+// lies in it, and once final goes back to dst once (if a match wrote into it) while its CRC32 is
+// taken from LDS. Windows of up to 64 matches whose output fits SPAN bytes are applied as in
+// inflate_resolve; a match source is read from the ring while the ring still holds it (at least
+// RING - SPAN - RES_FL bytes back from the window) and from dst before that (final bytes written
+// back earlier). Staging stretch s reuses the slot of stretch s - RING, which is final by then:
+// every byte before the window is, and RING >= SPAN + RES_FL keeps s - RING + RES_FL <= the
+// window's start.
+// CRC32: lane j takes bytes [32j, 32j + 32) of each stretch and keeps A_j = A_j * x^(8 RES_FL) xor
+// crc(piece) (the multiplication by table, kCrcAdv); the last, partial stretch is cut into 32-byte
+// pieces counted back from the block's end, and the block's CRC is xor_j x^(8 * 32 (63 - j)) *
+// (A_j * x^(8 r) xor c_j), r being the bytes of the partial stretch (zlib's crc32_combine rule).
+constexpr int RES_FL = 2048;
+// 16 bytes as a plain vector (loads and stores in an explicit address space)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr uint32_t crc_mulmod_c(uint32_t a, uint32_t b) {
+    uint32_t p = 0;
+    for (int k = 0; k < 32; k++) {
+        if (a & 0x80000000u) p ^= b;
+        a <<= 1;
+        b = (b & 1) ? (b >> 1) ^ 0xEDB88320u : b >> 1;
+    }
+    return p;
+}
+// kCrcAdv.t[k][v] = (v << 8k) * x^(8 * RES_FL) mod P: a CRC value times x^(8 RES_FL) is the xor of
+// four lookups, one per byte
+struct CrcAdv {
+    uint32_t t[4][256];
+};
+constexpr CrcAdv make_crc_adv() {
+    uint32_t x = 0x40000000u;                                // x^1
+    for (int j = 0; j < 14; j++) x = crc_mulmod_c(x, x);     // x^(2^14) = x^(8 * 2048)
+    CrcAdv c{};
+    for (int k = 0; k < 4; k++)
+        for (int v = 0; v < 256; v++) c.t[k][v] = crc_mulmod_c((uint32_t)v << (8 * k), x);
+    return c;
+}
+static_assert(RES_FL == 2048, "kCrcAdv is built for 2,048-byte stretches");
+__constant__ CrcAdv kCrcAdv = make_crc_adv();
+
+// zlib CRC32 of r[s, e) (LDS, e - s <= 32 or any length, s <= e), 4 bytes a step where aligned
+__device__ __forceinline__ uint32_t crc_lds(const CBM_LDS uint8_t* r, int s, int e, const uint32_t (*T)[256]) {
+    if (e <= s) return 0;
+    uint32_t x = 0xFFFFFFFFu;
+    int k = s;
+    for (; k < e && (k & 3); k++) x = T[0][(x ^ r[k]) & 255] ^ (x >> 8);
+    for (; k + 4 <= e; k += 4) {
+        x ^= *(const CBM_LDS uint32_t*)(r + k);
+        x = T[3][x & 255] ^ T[2][(x >> 8) & 255] ^ T[1][(x >> 16) & 255] ^ T[0][x >> 24];
+    }
+    for (; k < e; k++) x = T[0][(x ^ r[k]) & 255] ^ (x >> 8);
+    return x ^ 0xFFFFFFFFu;
+}
+
+template <int SPAN, int WPG, int RING>
+struct RingSmem {
+    alignas(16) uint8_t ring[WPG][RING];
+    uint32_t pend[WPG][SPAN / 32 + 2];
+};
+
+template <int SPAN, int WPG, int RING>
+__global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve_ring(uint8_t* dst, const int64_t* out_off,
+                                                                       const int32_t* isize, const uint32_t* expect,
+                                                                       uint32_t* crc_out, int32_t* status,
+                                                                       int64_t* bad, const uint64_t* pool,
+                                                                       const int32_t* mcount, int64_t n_blocks) {
+    static_assert(SPAN >= 258 && SPAN % 32 == 0, "a window holds one match at least");
+    static_assert((RING & (RING - 1)) == 0 && RING % RES_FL == 0 && RING >= SPAN + RES_FL,
+                  "the ring holds the window and the stretch staged past it");
+    static_assert(RING / RES_FL <= 32, "dirty stretches are bits of a word");
+    constexpr int MASK = RING - 1, NSLOT = RING / RES_FL;
+    __shared__ RingSmem<SPAN, WPG, RING> s;
+    const int wv = (int)threadIdx.x / CBM_WAVE, lane = (int)threadIdx.x % CBM_WAVE;
+    CBM_LDS uint8_t* const ring = (CBM_LDS uint8_t*)s.ring[wv];
+    uint32_t* pend = s.pend[wv];
+    const uint32_t(*T)[256] = kCrcTab.t;
+    const uint32_t(*TX)[256] = kCrcAdv.t;
+    // this lane's factor x^(8 * 32 * (63 - lane)), and x^(8 * 2^lane) for the partial stretch
+    uint32_t P, E;
+    {
+        uint32_t sq = 0x40000000u;
+        for (int j = 0; j < 8; j++) sq = crc_mulmod(sq, sq);  // x^256
+        uint32_t r = 0x80000000u;
+        for (int k = CBM_WAVE - 1 - lane; k; k >>= 1) {
+            if (k & 1) r = crc_mulmod(r, sq);
+            sq = crc_mulmod(sq, sq);
+        }
+        P = r;
+        uint32_t e = 0x40000000u;
+        for (int j = 0; j < 3; j++) e = crc_mulmod(e, e);  // x^8
+        for (int j = 0; j < lane && j < 12; j++) e = crc_mulmod(e, e);
+        E = e;
+    }
+    for (int64_t b = (int64_t)blockIdx.x * WPG + wv; b < n_blocks; b += (int64_t)gridDim.x * WPG) {
+        if (status[b] != CBM_BLOCK_OK) continue;  // left to the one-phase kernel
+        const int M = mcount[b], L = isize[b];
+        uint8_t* out = dst + out_off[b];
+        const int a0 = (int)((uintptr_t)out & 15);
+        CBM_GLOBAL uint8_t* const oal = (CBM_GLOBAL uint8_t*)(out - a0);  // 16-byte aligned: byte p is oal[a0 + p]
+        const int uend = a0 + L;
+        const uint64_t* ml = pool + (size_t)b * MCAP_TOK;
+        int st = 0, fin = 0;  // staged and written-back ends (u, multiples of RES_FL)
+        uint32_t dirty = 0;   // ring slots holding match output not yet written back
+        bool wb = false;      // written back since the last fence
+        uint32_t acc = 0;
+        // stretch [fin, fin + RES_FL) (final, wholly inside the block's end): CRC piece, write-back
+        auto flush = [&]() {
+            const CBM_LDS uint8_t* r = ring + (fin & MASK);
+            const int p0 = 32 * lane, ps = fin + p0 < a0 ? a0 - fin : p0;
+            const uint32_t c = crc_lds(r, ps, p0 + 32, T);
+            acc = TX[0][acc & 255] ^ TX[1][(acc >> 8) & 255] ^ TX[2][(acc >> 16) & 255] ^ TX[3][acc >> 24] ^ c;
+            const int slot = (fin / RES_FL) & (NSLOT - 1);
+            if ((dirty >> slot) & 1u) {
+#pragma unroll
+                for (int h = 0; h < RES_FL / 16 / CBM_WAVE; h++) {
+                    const int q = lane + h * CBM_WAVE, u = fin + 16 * q;
+                    if (u >= a0 && u + 16 <= uend) {
+                        *(CBM_GLOBAL u32x4*)(oal + u) = *(const CBM_LDS u32x4*)(r + 16 * q);
+                    } else {
+                        for (int i = 0; i < 16; i++)
+                            if (u + i >= a0 && u + i < uend) oal[u + i] = r[16 * q + i];
+                    }
+                }
+                dirty &= ~(1u << slot);
+                wb = true;
+            }
+            fin += RES_FL;
+        };
+        // stage whole stretches until [.., target) is in the ring. The stretch after the staged
+        // ones is loaded into registers ahead (nx, for nx_at), so that its latency passes while
+        // the windows before it are applied; its bytes are literals, final since the decode phase.
+        constexpr int PPL = RES_FL / 16 / CBM_WAVE;  // 16-byte pieces a lane
+        u32x4 nx[PPL];
+        int nx_at = -1;
+        auto prefetch = [&]() {
+            nx_at = st;
+#pragma unroll
+            for (int h = 0; h < PPL; h++) {
+                const int u = st + 16 * (lane + h * CBM_WAVE);
+                nx[h] = u < uend ? *(const CBM_GLOBAL u32x4*)(oal + u) : u32x4{0u, 0u, 0u, 0u};
+            }
+        };
+        auto stage_to = [&](int target) {
+            if (st >= target) return;
+            while (st < target) {
+                while (fin <= st - RING) flush();  // the slot's previous stretch goes back first
+                CBM_LDS uint8_t* r = ring + (st & MASK);
+                if (nx_at != st) prefetch();
+#pragma unroll
+                for (int h = 0; h < PPL; h++) *(CBM_LDS u32x4*)(r + 16 * (lane + h * CBM_WAVE)) = nx[h];
+                st += RES_FL;
+            }
+            if (st < uend) prefetch();
+            res_wave_fence();
+        };
+        bool stuck = false;
+        uint64_t m = lane < M ? ml[lane] : 0;  // this window's matches (the next window's load ahead)
+        for (int k0 = 0; k0 < M && !stuck;) {
+            const int k = k0 + lane;
+            const int dest = (int)(m & 0x1ffff), dist = (int)((m >> 17) & 0xffff), len = (int)((m >> 33) & 0x1ff);
+            const int lo = __builtin_amdgcn_readlane(dest, 0);
+            // dest + len grows with the match index: the matches that fit are a prefix
+            const int c = __popcll(__ballot(k < M && dest + len - lo <= SPAN));
+            const int hi = __builtin_amdgcn_readlane(dest + len, c - 1);
+            const int ulo = a0 + lo, uhi = a0 + hi;
+            const uint64_t mn = k0 + c + lane < M ? ml[k0 + c + lane] : 0;
+            stage_to(uhi);
+            const int rbase = st - RING;  // the ring holds [rbase, st); below that, dst (final)
+            for (int w = lane; w <= ((uhi - ulo) >> 5); w += CBM_WAVE) pend[w] = 0;
+            dirty |= ((2u << ((uhi - 1) / RES_FL - ulo / RES_FL)) - 1u) << ((ulo / RES_FL) & (NSLOT - 1));
+            dirty |= dirty >> NSLOT;  // slots wrap
+            dirty &= (NSLOT == 32 ? 0xffffffffu : (1u << NSLOT) - 1u);
+            const bool mine = lane < c;
+            const int usrc = a0 + dest - dist, udst = a0 + dest;
+            // a source wholly before the window (and not overlapping its output) is final: copy it now
+            const bool now = mine && dist >= len && dest - dist + len <= lo;
+            if (wb && __ballot(now && usrc < rbase)) {
+                res_fence();  // write-backs land before they are read
+                wb = false;
+            }
+            if (now) {
+                for (int i0 = 0; i0 < len; i0 += 32) {
+                    const int us = usrc + i0, sh = us & 3, base = us - sh;
+                    const int nb = len - i0 < 32 ? len - i0 : 32;
+                    const int nw = (nb + sh + 3) >> 2;
+                    uint32_t x[9];
+#pragma unroll
+                    for (int j = 0; j < 9; j++) {
+                        const int wpos = base + 4 * j;
+                        uint32_t v = 0u;
+                        if (j < nw) {
+                            if (wpos < rbase) v = *(const CBM_GLOBAL uint32_t*)(oal + wpos);
+                            else v = *(const CBM_LDS uint32_t*)(ring + (wpos & MASK));
+                        }
+                        x[j] = v;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const uint32_t v = __builtin_amdgcn_alignbyte(x[j + 1], x[j], (uint32_t)sh);
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+                            if (4 * j + t < nb) ring[(udst + i0 + 4 * j + t) & MASK] = (uint8_t)(v >> (8 * t));
+                    }
+                }
+            }
+            const bool near = mine && !now;
+            res_wave_fence();
+            if (near) mark_bits(pend, udst - ulo, len, true);
+            res_wave_fence();
+            bool left = near;
+            int idle = 0;
+            while (__ballot(left)) {
+                bool rdy = false;
+                if (left) {
+                    const int e = usrc + (dist < len ? dist : len), a = usrc < ulo ? ulo : usrc;
+                    rdy = !(a < e && any_bits(pend, a - ulo, e - ulo));
+                }
+                const uint64_t rm0 = __ballot(rdy);
+                if (!rm0 && ++idle > (1 << 12)) {  // cannot happen (the lowest is ready):
+                    stuck = true;                   // leave the block to the one-phase kernel
+                    break;
+                }
+                if (wb && rm0) {
+                    res_fence();
+                    wb = false;
+                }
+                // each ready match copied by the whole wavefront, a byte a lane (sources final,
+                // destinations disjoint and pending); an overlapping match repeats its dist bytes
+                for (uint64_t rm = rm0; rm; rm &= rm - 1) {
+                    const int l = __builtin_ctzll(rm);
+                    const int md = __builtin_amdgcn_readlane(udst, l);
+                    const int dl = __builtin_amdgcn_readlane(dist, l);
+                    const int ln = __builtin_amdgcn_readlane(len, l);
+                    const int sr = md - dl;
+                    for (int i = lane; i < ln; i += CBM_WAVE) {
+                        const int q = sr + (dl >= ln ? i : (int)((uint32_t)i % (uint32_t)dl));
+                        uint8_t v;
+                        if (q < rbase) v = oal[q];
+                        else v = ring[q & MASK];
+                        ring[(md + i) & MASK] = v;
+                    }
+                }
+                res_wave_fence();  // the bytes land before their bits clear
+                if (rdy) mark_bits(pend, udst - ulo, len, false);
+                res_wave_fence();
+                left = left && !rdy;
+            }
+            k0 += c;
+            m = mn;
+        }
+        if (stuck) {
+            if (lane == 0) status[b] = BLOCK_DEFER;
+            continue;
+        }
+        // the rest of the block: stage, then every whole stretch, then the partial one
+        stage_to(uend);
+        while (fin + RES_FL <= uend) flush();
+        const int f0 = fin > a0 ? fin : a0;
+        const int r = uend - f0 > 0 ? uend - f0 : 0;
+        uint32_t cj = 0;
+        if (r) {
+            const CBM_LDS uint8_t* rr = ring + (fin & MASK);
+            const int e = uend - 32 * (CBM_WAVE - 1 - lane), sb = e - 32 > f0 ? e - 32 : f0;
+            cj = crc_lds(rr, sb - fin, e - fin, T);
+            const int slot = (fin / RES_FL) & (NSLOT - 1);
+            if ((dirty >> slot) & 1u)
+                for (int i = lane; i < uend - fin; i += CBM_WAVE)
+                    if (fin + i >= a0) oal[fin + i] = rr[i];
+        }
+        uint32_t xr = 0x80000000u;  // x^(8 r)
+        for (int j = 0; j < 12; j++)
+            if ((r >> j) & 1) xr = crc_mulmod(xr, (uint32_t)__builtin_amdgcn_readlane((int)E, j));
+        uint32_t cr = crc_mulmod(P, crc_mulmod(acc, xr) ^ cj);
+        for (int d = 32; d; d >>= 1) cr ^= __shfl_xor(cr, d, CBM_WAVE);
+        if (lane == 0) {
+            crc_out[b] = cr;
+            if (cr != expect[b]) {
+                status[b] = CBM_BLOCK_CRC;
+                atomicAdd((unsigned long long*)bad, 1ull);
+            }
+        }
+    }
+}
+
 // the blocks left to the one-phase kernel, listed (in any order) with their count
 __global__ void inflate_defer_list(const int32_t* status, int64_t n_blocks, int64_t* idx, int64_t* count) {
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n_blocks; b += (int64_t)gridDim.x * blockDim.x)

@@ -6,15 +6,20 @@ The inflate kernel comes in launch shapes (``cbm_inflate_tune``):
   with SEG bits per thread and round. NT and SEG set the LDS layout: the
   compressed-bit staging area is NT * SEG / 8 bytes beside the 64 KiB output
   window and the decode tables.
+* two-phase variants (a wavefront per block decodes SEG-bit segments into literals
+  and a match list; a resolve pass applies the matches): ``blocks_per_wave`` picks
+  the resolve pass's LDS layout (a window of 64 or 128 matches, or a history ring
+  of 4 to 32 KiB a wavefront) and ``waves_per_group`` its blocks per workgroup.
 * the lane variant, where every lane of a wavefront inflates a block of its own,
   with ``blocks_per_wave`` x ``waves_per_group`` blocks per workgroup.
 
 :func:`autotune` times every candidate on a sample of a batch's blocks and keeps
 the fastest per (GPU, batch-size bucket) in a JSON cache
 (``$CUBAM_AMD_TUNE_CACHE``, default ``~/.cache/cubam_amd/inflate_tune.json``).
-:func:`lookup` returns the cached choice for a batch, and
-:func:`cubam_amd._decode.inflate_blocks` uses it once :func:`enable` has been
-called (or ``CUBAM_AMD_AUTOTUNE=1`` is set).
+:func:`lookup` returns the cached choice for a batch, and every inflate
+(:func:`cubam_amd._decode.inflate_blocks`, so ``read_bam`` and the serving loop)
+uses it when one is cached for the batch's bucket on this GPU; ``enable(False)``
+or ``CUBAM_AMD_AUTOTUNE=0`` turns that off (the library default is used).
 
 :class:`GraphInflate` captures the inflate of one fixed set of blocks (a
 batched heterogeneous decode) in a hipGraph, so each later decode of new bytes
@@ -33,14 +38,20 @@ import torch
 
 from cubam_amd import _decode, _lib
 
-#: workgroup variants, then the lane variant at 1, 2, 4 and 8 blocks per workgroup
+#: workgroup variants; the two-phase variants by segment bits, resolve layout (window of 64 or 128
+#: matches, or a history ring of 4 to 32 KiB) and resolve blocks per workgroup; then the lane
+#: variant at 1, 2, 4 and 8 blocks per workgroup
 CANDIDATES = tuple(
     [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13)]
+    + [(11, 1, wpg, 0) for wpg in (2, 8)]
+    + [(11, 2, wpg, 0) for wpg in (2, 4)]
+    + [(11, ring, wpg, 0) for ring, wpg in ((4, 1), (4, 4), (8, 1), (8, 2), (16, 1), (32, 1))]
+    + [(v, 1, 2, 0) for v in (12, 13)]
     + [(_lib.INFLATE_LANES, bpw, 1, 0) for bpw in (1, 2, 4, 8)]
     + [(_lib.INFLATE_LANES, 4, 2, 0), (_lib.INFLATE_LANES, 8, 2, 0)]
 )
 
-_enabled = os.environ.get("CUBAM_AMD_AUTOTUNE", "") not in ("", "0")
+_enabled = os.environ.get("CUBAM_AMD_AUTOTUNE", "1") != "0"
 _cache: dict | None = None
 
 

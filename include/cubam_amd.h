@@ -112,11 +112,13 @@ enum {
 typedef struct {
     int32_t variant; /* CBM_INFLATE_* */
     /* CBM_INFLATE_LANES: lanes of a wavefront that each inflate a block, 1..64 (default 4);
-       two-phase variants: matches a lane of the resolve pass takes per window, 1, 2 or 4 (default 1) */
+       two-phase variants, the resolve pass's LDS layout: 1 or 2 = matches a lane per window of
+       2 or 4 KiB (default 1); 4, 8, 16 or 32 = a history ring of that many KiB a wavefront, which
+       takes each 2 KiB of output through LDS once and fuses the CRC32 into its write-back */
     int32_t blocks_per_wave;
     /* CBM_INFLATE_LANES: wavefronts per workgroup, 1..16 (default 2); two-phase variants: blocks
-       (one a wavefront) per workgroup of the resolve pass, 2, 4 or 8 (default 4; 4 only with 4
-       matches a lane) */
+       (one a wavefront) per workgroup of the resolve pass: 2, 4 or 8 with 1 match a lane (default
+       4), 2 or 4 with 2; with a ring, 1, 2 or 4 (4 KiB, 8 KiB), 1 or 2 (16 KiB), 1 (32 KiB) */
     int32_t waves_per_group;
     int32_t groups; /* workgroups of the (first) decode launch; 0 = as many as the CUs hold */
 } cbm_inflate_tune;

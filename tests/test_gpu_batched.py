@@ -52,13 +52,15 @@ def test_autotune_picks_and_caches(hetero, tmp_path, monkeypatch):
     assert (tmp_path / "t.json").exists()
     got = tune.lookup(blocks.n_blocks, blocks.total_out)
     assert tuple((got.variant, got.blocks_per_wave, got.waves_per_group, got.groups)) == best
+    was = tune.enabled()
     tune.enable(True)
     try:
+        assert _decode.inflate_tune_for(blocks.n_blocks, blocks.total_out) is not None  # the cached shape
         inf = _decode.inflate_blocks(blocks, None)
         _decode.check_inflated(inf)
         _check(inf.data, inf.crc, inf.status, hb, crc)
     finally:
-        tune.enable(False)
+        tune.enable(was)
         monkeypatch.setattr(tune, "_cache", None)
 
 

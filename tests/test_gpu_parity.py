@@ -73,6 +73,9 @@ def test_hetero_blocks_bytes_and_crc(cbm, oracle):
                                                     (4, 0, 0, 0), (5, 0, 0, 0), (6, 0, 0, 0), (7, 0, 0, 0),
                                                     (8, 0, 0, 0), (9, 0, 0, 0), (10, 0, 0, 0), (10, 0, 0, 3),
                                                     (11, 0, 0, 0), (11, 0, 0, 5), (12, 0, 0, 0), (13, 0, 0, 0),
+                                                    (11, 2, 2, 0), (11, 4, 1, 0), (11, 4, 4, 0), (11, 8, 1, 0), (11, 8, 2, 0), (11, 8, 4, 0),
+                                                    (11, 16, 1, 0), (11, 16, 2, 0), (11, 32, 1, 0), (12, 8, 4, 0),
+                                                    (13, 16, 2, 0),
                                                     (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0), (1, 32, 1, 0)])
 def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     from cubam_amd import _lib
@@ -335,9 +338,9 @@ def test_crc32_spans_matches_zlib(cbm):
     assert got == [zlib.crc32(raw[o : o + s].tobytes()) for o, s in zip(off, size)]
 
 
-@pytest.mark.parametrize("variant", [0, 11])
+@pytest.mark.parametrize("variant,bpw,wpg", [(0, 0, 0), (11, 1, 4), (11, 4, 2), (11, 8, 4), (11, 16, 2)])
 @pytest.mark.parametrize("where", ["header", "anywhere"])
-def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant):
+def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant, bpw, wpg):
     """Blocks with a few bits flipped in their dynamic Huffman header (the code-length code and
     the code lengths: "header") or anywhere in the compressed data, with the trailer of the
     intact payload, among intact blocks. Against zlib on
@@ -372,8 +375,8 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant):
         blob += block
     blob += bf.EOF_BLOCK
     arr = np.frombuffer(bytes(blob), np.uint8)
-    _say("corrupt blocks:", where, len(off), "blocks, variant", variant)
-    blocks, inf = _gpu_blocks(arr, _lib.InflateTune(variant, 0, 0, 0))
+    _say("corrupt blocks:", where, len(off), "blocks, shape", variant, bpw, wpg)
+    blocks, inf = _gpu_blocks(arr, _lib.InflateTune(variant, bpw, wpg, 0))
     assert blocks.n_blocks == len(off) + 1  # and the EOF block
     st = inf.status.cpu().numpy()[: len(off)] & 255
     data = inf.data[: inf.n].cpu().numpy()
@@ -392,8 +395,8 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant):
         o += isz[k]
 
 
-@pytest.mark.parametrize("variant", [11, 12, 13])
-def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant):
+@pytest.mark.parametrize("variant,bpw,wpg", [(11, 0, 0), (12, 0, 0), (13, 0, 0), (11, 8, 4), (11, 16, 2)])
+def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant, bpw, wpg):
     """Blocks with more matches than the decode phase's list holds (3-byte matches all through),
     stored blocks, fixed-Huffman blocks, and ordinary ones, mixed: the two-phase inflate leaves the
     first kind to the one-phase kernel and every block comes out as zlib makes it, CRC32 included."""
@@ -416,7 +419,7 @@ def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant):
         payloads.append(p)
         blob += bf.bgzf_block(p, lvl, strat)
     blob += bf.EOF_BLOCK
-    blocks, inf = _gpu_blocks(np.frombuffer(blob, np.uint8), _lib.InflateTune(variant, 0, 0, 0))
+    blocks, inf = _gpu_blocks(np.frombuffer(blob, np.uint8), _lib.InflateTune(variant, bpw, wpg, 0))
     assert int(inf.result.item()) == 0
     assert (inf.status.cpu().numpy() == 0).all()
     assert inf.data[: inf.n].cpu().numpy().tobytes() == b"".join(payloads)

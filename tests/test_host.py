@@ -45,6 +45,11 @@ def test_candidates_cover_the_launch_shapes():
     assert workgroup == {v for v in range(_lib.INFLATE_VARIANTS) if v != _lib.INFLATE_LANES}
     lanes = {(c[1], c[2]) for c in tune.CANDIDATES if c[0] == _lib.INFLATE_LANES}
     assert {bpw for bpw, _ in lanes} == {1, 2, 4, 8}  # blocks per workgroup {1, 2, 4, 8}
+    # two-phase: the resolve pass's LDS layout (window of 64 or 128 matches, history ring of 4 to
+    # 32 KiB) and its blocks per workgroup
+    two = {(c[1], c[2]) for c in tune.CANDIDATES if c[0] in _lib.INFLATE_TWO_PHASE}
+    assert {bpw for bpw, _ in two} == {0, 1, 2, 4, 8, 16, 32}
+    assert {wpg for _, wpg in two} >= {0, 1, 2, 4, 8}
 
 
 def test_write_framing_constants():
