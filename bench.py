@@ -19,6 +19,7 @@ closed-loop throughput and p50/p99 latency at swept Poisson arrival rates.
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -348,6 +349,8 @@ def serve_bench(gex_records: int, atac_records: int, n_requests: int, rates, bud
     budget = int(budget_gb * 1e9)
 
     def run(mode, trace):
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         with Server(srcs, budget_bytes=budget, mode=mode) as srv:
             reqs = run_trace(srv, trace)
             out = summarize(reqs)
@@ -423,6 +426,7 @@ def main():
                     help="compressor of the serving files (libdeflate: same level, about 3x faster to generate)")
     ap.add_argument("--serve-requests", type=int, default=160)
     args = ap.parse_args()
+    faulthandler.enable()  # a crash prints every thread's Python stack
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -521,6 +525,8 @@ def main():
         del wbuf
     if not args.no_batched:
         res["batched"] = batched_bench(args.batched_blocks, max(3, min(args.steps, 10)))
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()  # the serving runs start from the same allocator state every time
     if not args.no_serve:
         res["serving"] = serve_bench(args.serve_gex_records, args.serve_atac_records, args.serve_requests,
                                      rates=(0.5, 1.0, 2.0), codec=args.serve_codec)
