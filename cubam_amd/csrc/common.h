@@ -99,12 +99,14 @@ __device__ inline bool is_record_at(const A& a, int64_t n, int64_t o, int n_ref)
     // CBM_PAD keeps readable)
     const uint32_t nn = h.lrn - 1;
     for (uint32_t i = 0; i < nn; i += 4) {
-        const uint32_t w = a.u32(nm + i);
-#pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t c = (w >> (8 * k)) & 255;
-            if (i + k < nn && (c < 33 || c > 126 || c == '@')) return false;
-        }
+        const uint32_t w = a.u32(nm + i), v = w ^ 0x40404040u;
+        // bit 7 of each byte: below 33, above 126, or '@' (per-byte subtraction and addition
+        // across the word: a borrow or carry reaches a byte only above a byte that fails the same
+        // test, so some name byte fails if and only if a name byte's bit is set)
+        const uint32_t bad = (((w - 0x21212121u) & ~w) | ((w + 0x01010101u) | w) | ((v - 0x01010101u) & ~v)) &
+                             0x80808080u;
+        const uint32_t keep = nn - i >= 4 ? 0xffffffffu : (1u << (8 * (nn - i))) - 1u;
+        if (bad & keep) return false;
     }
     int64_t cg = nm + h.lrn, rlen = 0;
     for (uint32_t i = 0; i < h.ncig; i++) {
