@@ -123,6 +123,28 @@ class BlockIndex:
         c1 = int(self.coff[k1 - 1] + self.bsize[k1 - 1])
         return Span(k0, k1, s, e, r0, r1, int(self.coff[k0]), c1, int(self.uoff[k0]))
 
+    def voffset_span(self, vbeg: int, vend: int) -> Span:
+        """The span of the records from virtual offset ``vbeg`` (a record start) to ``vend`` (a
+        record end), as a BAI chunk names them; its record count is unknown (rec0 = rec1 = -1)."""
+        def stream_at(v: int) -> tuple[int, int]:
+            c, u = v >> 16, v & 0xFFFF
+            k = int(np.searchsorted(self.coff, c))
+            if k < self.coff.size and int(self.coff[k]) == c:
+                return k, int(self.uoff[k]) + u
+            last = self.coff.size - 1
+            if last >= 0 and u == 0 and c == int(self.coff[last] + self.bsize[last]):
+                return last + 1, int(self.uoff[last] + self.isize[last])
+            raise ValueError(f"virtual offset {v:#x} is not at a block of this file")
+
+        k0, start = stream_at(vbeg)
+        _, end = stream_at(vend)
+        if not (k0 < self.coff.size and start < end <= self.stream_bytes):
+            raise ValueError(f"virtual offsets [{vbeg:#x}, {vend:#x}) name no records of this file")
+        ends = self.uoff + self.isize
+        k1 = max(int(np.searchsorted(ends, end - 1, side="right")) + 1, k0 + 1)
+        c1 = int(self.coff[k1 - 1] + self.bsize[k1 - 1])
+        return Span(k0, k1, start, end, -1, -1, int(self.coff[k0]), c1, int(self.uoff[k0]))
+
     def _keys(self, rb: np.ndarray) -> np.ndarray:
         ref = self.first_ref[rb].astype(np.int64)
         ref = np.where(ref < 0, len(self.header.names), ref)  # unplaced reads sort last
@@ -174,8 +196,13 @@ def _le32(data: torch.Tensor, o: torch.Tensor) -> torch.Tensor:
     return torch.where(v >= (1 << 31), v - (1 << 32), v).to(torch.int32)
 
 
-def build_index(src, chunk_bytes: int = 256 << 20, ws: _decode.Workspace | None = None) -> BlockIndex:
-    """Decode a BAM file (a path or a host byte array) chunk by chunk on the GPU and index it."""
+def build_index(src, chunk_bytes: int = 256 << 20, ws: _decode.Workspace | None = None,
+                on_records=None) -> BlockIndex:
+    """Decode a BAM file (a path or a host byte array) chunk by chunk on the GPU and index it.
+
+    ``on_records(data, offsets, stream_offsets, chain)``, if given, is called for each chunk's
+    records (device tensors: the chunk's inflated data, the records' offsets in it and in the
+    file's inflated stream), in file order."""
     raw = host_bytes(src)
     header = header_of(src)
     n_ref = len(header.names)
@@ -227,6 +254,8 @@ def build_index(src, chunk_bytes: int = 256 << 20, ws: _decode.Workspace | None 
         if chain.n_rec:
             off = _decode.record_offsets(data, start, n, chain)
             stream = off + data_base
+            if on_records is not None:
+                on_records(data, off, stream, chain)
             blk = torch.searchsorted(starts_dev, stream, right=True) - 1
             new = torch.ones_like(blk, dtype=torch.bool)
             new[1:] = blk[1:] != blk[:-1]

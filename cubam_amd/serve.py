@@ -309,7 +309,7 @@ def decode_plain(src: BamSource, sp: Span, tags=("CB", "UB"), ws: _decode.Worksp
     inf = _decode.inflate_blocks(blocks, None, ws)
     start, end = sp.start - sp.u0, sp.end - sp.u0
     chain = _decode.record_chain(inf.data, start, end, src.n_ref, ws, inf=inf)
-    if chain.tail != end or chain.n_rec != sp.records:
+    if chain.tail != end or (sp.rec1 >= 0 and chain.n_rec != sp.records):  # rec1 < 0: count unknown
         raise ValueError(_decode.MSG_NO_BLOCK)
     off, csr = _decode.record_offsets(inf.data, start, end, chain, csr=True)
     return RecordBatch.decode(inf.data, off, src.header, chain, tags, csr=csr)
