@@ -150,8 +150,10 @@ int cbm_inflate(const uint8_t* src_d, const int64_t* blk_off_d, const int32_t* b
  * until the process ends, found once per device with hipGetSymbolAddress (the
  * calling thread's current device must be the stream's). It holds 512 match
  * lists, so tune->groups above 512 is rejected (CBM_ERR_ARG); the default
- * launch uses min(512, two a CU). Results are those of cbm_inflate with the
- * same tune. Every call on a GPU shares that GPU's array, and each call clears
+ * launch uses min(512, two a CU). A null or two-phase tune runs the one-phase
+ * kernel (CBM_INFLATE_WG512_S128), whose workspace fits the array; results
+ * (bytes, CRC32s, statuses) are those of cbm_inflate, every variant giving the
+ * same. Every call on a GPU shares that GPU's array, and each call clears
  * what must start at zero: calls that overlap on one GPU (two streams, two
  * threads) race and must be serialised by the caller.
  */
