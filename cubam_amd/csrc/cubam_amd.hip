@@ -181,8 +181,8 @@ static int repair_store(const RecWs& w, std::vector<int64_t>& f, const std::vect
 // workgroups of the persistent inflate launch: the LDS of a CU holds two block buffers
 static int64_t inflate_max_groups() { return (int64_t)device_cus() * 2; }
 
-// one launch of inflate_wave<NT, SEG, TOK>: a persistent grid taking blocks from next_block
-template <int NT, int SEG, bool TOK>
+// one launch of inflate_wave<NT, SEG, TOK, FUSE>: a persistent grid taking blocks from next_block
+template <int NT, int SEG, bool TOK, bool FUSE = false>
 static int launch_inflate_wave(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                                const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
                                uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t max_groups,
@@ -191,7 +191,7 @@ static int launch_inflate_wave(const uint8_t* src, const int64_t* blk_off, const
     const size_t lds = sizeof(WaveSmem<NT, SEG, TOK>);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void*)inflate_wave<NT, SEG, TOK>,
+        (void)hipFuncSetAttribute((const void*)inflate_wave<NT, SEG, TOK, FUSE>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
@@ -203,7 +203,7 @@ static int launch_inflate_wave(const uint8_t* src, const int64_t* blk_off, const
     if (g > max_groups) g = max_groups;  // the workspace holds that many match lists (one-phase)
     if (g > n_blocks) g = n_blocks;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL((inflate_wave<NT, SEG, TOK>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
+    hipLaunchKernelGGL((inflate_wave<NT, SEG, TOK, FUSE>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
                        crc_expect, out_off, n_blocks, dst, crc, status, result, (unsigned long long*)next_block,
                        mlists, (unsigned long long*)stats, idx, n_dev, pool, mcount);
     return check_launch("inflate_wave");
@@ -256,7 +256,7 @@ static int launch_resolve_ring(uint8_t* dst, const int64_t* out_off, const int32
     return check_launch("inflate_resolve_ring");
 }
 
-template <int SEG>
+template <int SEG, bool FUSE = false>
 static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
                       const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
                       uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int res_mpl, int res_wpg,
@@ -267,11 +267,13 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
     int64_t* idx = (int64_t*)tok;
     int32_t* mcount = (int32_t*)(idx + n_blocks);
     uint64_t* pool = (uint64_t*)(tok + ((12 * n_blocks + 255) / 256) * 256);
-    int rc = launch_inflate_wave<64, SEG, true>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
-                                                status, result, groups, (int64_t)1 << 40, counter, nullptr,
-                                                counter + 8, nullptr, nullptr, pool, mcount, st);
+    int rc = launch_inflate_wave<64, SEG, true, FUSE>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst,
+                                                      crc, status, result, groups, (int64_t)1 << 40, counter,
+                                                      nullptr, counter + 8, nullptr, nullptr, pool, mcount, st);
     if (rc) return rc;
-    switch (res_mpl * 100 + res_wpg) {
+    switch (FUSE ? 0 : res_mpl * 100 + res_wpg) {
+        case 0:  // resolved by the decode launch
+            break;
 #define RES_CASE(MPL_, WPG_)                                                                                     \
     case MPL_ * 100 + WPG_:                                                                                      \
         rc = launch_resolve<MPL_, 2048 * MPL_, WPG_>(dst, out_off, isize, crc_expect, crc, status, result, pool, \
@@ -428,7 +430,7 @@ int64_t cbm_inflate_workspace(int64_t n_blocks) {
 
 }  // extern "C"
 
-static bool tok_variant(int v) { return v >= CBM_INFLATE_TOK64_S512 && v <= CBM_INFLATE_TOK64_S256; }
+static bool tok_variant(int v) { return v >= CBM_INFLATE_TOK64_S512 && v <= CBM_INFLATE_TOK64_S512_FUSED; }
 
 // cbm_inflate's work once the workspace is settled: ws holds INFLATE_WS_HDR bytes and max_groups
 // match lists (unused by the lane variant), and with `tok` the two-phase inflate's tok_ws_bytes
@@ -475,6 +477,10 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
             TOK_CASE(CBM_INFLATE_TOK64_S1024, 1024)
             TOK_CASE(CBM_INFLATE_TOK64_S256, 256)
 #undef TOK_CASE
+            case CBM_INFLATE_TOK64_S512_FUSED:
+                return launch_tok<512, true>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                             status, result, groups, res_mpl, res_wpg, ws, inflate_max_groups(),
+                                             max_groups, st);
             default: ARG(false, "cbm_inflate: unknown variant");
         }
     }

@@ -561,10 +561,17 @@ __constant__ CrcTables kCrcTab = make_crc_tables();
 // before their segment.
 // idx / n_dev (may be null): decode blocks idx[0 .. *n_dev) instead of 0 .. n_blocks.
 constexpr int MCAP_TOK = 6144;
+template <int MPL, int SPAN>
+__device__ __forceinline__ void resolve_block(uint8_t* out, int L, uint32_t expect, uint32_t* crc_out_b,
+                                              int32_t* status_b, int64_t* bad, const uint64_t* ml, int M,
+                                              uint8_t* sp, uint32_t* pend, const uint32_t (*T)[256], uint32_t xk,
+                                              int lane);
 #ifndef CBM_TOK_OVL
 #define CBM_TOK_OVL 256
 #endif
-template <int NT, int SEG, bool TOK = false>
+// FUSE (with TOK): the wavefront resolves each block it decoded at once (resolve_block, in the
+// staging area, which is free until the next block stages), instead of a resolve launch after
+template <int NT, int SEG, bool TOK = false, bool FUSE = false>
 __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) inflate_wave(const uint8_t* src, const int64_t* blk_off,
                                                    const int32_t* bsize, const int32_t* isize,
                                                    const uint32_t* expect, const int64_t* out_off,
@@ -574,6 +581,9 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                                                    unsigned long long* stats, const int64_t* idx,
                                                    const int64_t* n_dev, uint64_t* pool, int32_t* mcount) {
     using S = WaveSmem<NT, SEG, TOK>;
+    static_assert(!FUSE || (TOK && NT == CBM_WAVE), "a fused resolve is one wavefront's");
+    static_assert(!FUSE || S::IN_WORDS * 4 >= 2048 + 48 + ((2048 + 16) / 32 + 2) * 4,
+                  "the staging area holds the resolve's window and bitmap");
     extern __shared__ __align__(16) uint8_t smem_raw[];
     S& s = *(S*)smem_raw;
     const int tid = threadIdx.x;
@@ -1315,7 +1325,15 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     for (int k = 0; k < STAT_N; k++) atomicAdd(&stats[k], (unsigned long long)acc[k]), acc[k] = 0;
 #endif
             }
-            __syncthreads();
+            __syncthreads();  // also: the block's literals and match list, written above, are visible
+            if constexpr (FUSE) {
+                if (err == CBM_BLOCK_OK) {
+                    uint8_t* const win = (uint8_t*)s.in;  // 16-byte aligned (the tables before it are)
+                    resolve_block<1, 2048>(bdst, cap, expect[b], crc_out + b, status + b, bad_count, blist,
+                                           s.sv[SV_MCNT], win, (uint32_t*)(win + 2048 + 48), kCrcTab.t, xfac, tid);
+                }
+                __syncthreads();  // the staging area is free again for the next block
+            }
             continue;
         }
         const int nout = err == CBM_BLOCK_OK ? cap : (op < cap ? op : cap);
@@ -1465,6 +1483,139 @@ __device__ __forceinline__ void res_copy_far(const uint8_t* g, uint8_t* d, int l
     }
 }
 
+// One block's resolve: its M matches (ml, in output order) applied to its L bytes at out, in
+// windows of up to 64 * MPL matches whose output fits SPAN bytes, then its CRC32 checked
+// against expect (into *crc_out_b, *status_b and the bad count). sp: SPAN + 48 bytes of LDS,
+// 16-byte aligned; pend: (SPAN + 16) / 32 + 2 words of LDS; T the slicing-by-4 tables; xk this
+// lane's x^(8 * 1024 * (63 - lane)) mod P. A window that cannot progress leaves the block to the
+// one-phase kernel (status BLOCK_DEFER).
+template <int MPL, int SPAN>
+__device__ __forceinline__ void resolve_block(uint8_t* out, int L, uint32_t expect, uint32_t* crc_out_b, int32_t* status_b,
+                              int64_t* bad, const uint64_t* ml, int M, uint8_t* sp, uint32_t* pend,
+                              const uint32_t (*T)[256], uint32_t xk, int lane) {
+    bool stuck = false;
+    for (int k0 = 0; k0 < M && !stuck;) {
+        int dest[MPL], dist[MPL], len[MPL];
+#pragma unroll
+        for (int u = 0; u < MPL; u++) {
+            const int k = k0 + u * CBM_WAVE + lane;
+            const uint64_t m = k < M ? ml[k] : 0;
+            dest[u] = (int)(m & 0x1ffff), dist[u] = (int)((m >> 17) & 0xffff), len[u] = (int)((m >> 33) & 0x1ff);
+        }
+        const int lo = __builtin_amdgcn_readlane(dest[0], 0);
+        // dest + len grows with the match index (outputs are disjoint and in order): the
+        // matches that fit are a prefix, c of them (match k0's at most 258 bytes always fits)
+        int c = 0;
+#pragma unroll
+        for (int u = 0; u < MPL; u++)
+            c += __popcll(__ballot(k0 + u * CBM_WAVE + lane < M && dest[u] + len[u] - lo <= SPAN));
+        int hi = 0;
+#pragma unroll
+        for (int u = 0; u < MPL; u++)
+            if ((c - 1) / CBM_WAVE == u) hi = __builtin_amdgcn_readlane(dest[u] + len[u], (c - 1) % CBM_WAVE);
+        const int nspan = hi - lo;
+        const int shift = (int)((uintptr_t)(out + lo) & 15);
+        // stage the span: the 16-byte aligned pieces that cover it
+        const uint8_t* g0 = out + lo - shift;
+        const int npiece = (shift + nspan + 15) >> 4;
+        for (int q = lane; q < npiece; q += CBM_WAVE) *(uint4*)(sp + 16 * q) = *(const uint4*)(g0 + 16 * q);
+        for (int w = lane; w <= ((shift + nspan) >> 5); w += CBM_WAVE) pend[w] = 0;
+        bool near[MPL];
+#pragma unroll
+        for (int u = 0; u < MPL; u++) {
+            const bool mine = u * CBM_WAVE + lane < c;
+            const int src = dest[u] - dist[u];
+            // far: needs nothing of the window, so its lane copies it at once
+            const bool far = mine && dist[u] >= len[u] && src + len[u] <= lo;
+            if (far) res_copy_far(out + src, sp + shift + dest[u] - lo, len[u]);
+            near[u] = mine && !far;
+        }
+        res_wave_fence();
+#pragma unroll
+        for (int u = 0; u < MPL; u++)
+            if (near[u]) mark_bits(pend, shift + dest[u] - lo, len[u], true);
+        res_wave_fence();
+        uint32_t left = 0;
+#pragma unroll
+        for (int u = 0; u < MPL; u++) left |= near[u] ? 1u << u : 0u;
+        int idle = 0;
+        while (__ballot(left != 0)) {
+            uint32_t rdy = 0;
+#pragma unroll
+            for (int u = 0; u < MPL; u++) {
+                if (!(left & (1u << u))) continue;
+                const int src = dest[u] - dist[u], e = src + (dist[u] < len[u] ? dist[u] : len[u]);
+                const int a0 = src < lo ? lo : src;
+                if (!(a0 < e && any_bits(pend, shift + a0 - lo, shift + e - lo))) rdy |= 1u << u;
+            }
+            if (!__ballot(rdy != 0) && ++idle > (1 << 12)) {  // cannot happen (the lowest is ready):
+                stuck = true;                                  // leave the block to the one-phase kernel
+                break;
+            }
+            // the ready matches, one at a time, each copied by the whole wavefront (a byte a
+            // lane): their destinations are disjoint and pending, their sources final, so they
+            // go in any order. Byte i of a match reads src + i, or src + i mod dist when it
+            // overlaps its own output (a repeat of the dist final bytes before dest).
+#pragma unroll
+            for (int u = 0; u < MPL; u++) {
+                for (uint64_t rm = __ballot((rdy >> u) & 1u); rm; rm &= rm - 1) {
+                    const int l = __builtin_ctzll(rm);
+                    const int md = __builtin_amdgcn_readlane(dest[u], l);
+                    const int dl = __builtin_amdgcn_readlane(dist[u], l);
+                    const int ln = __builtin_amdgcn_readlane(len[u], l);
+                    const int src = md - dl;
+                    uint8_t* d = sp + shift + md - lo;
+                    if (src >= lo) {  // the source is in the span: LDS only
+                        const uint8_t* sl = sp + shift + src - lo;
+                        if (dl >= ln) {
+                            for (int i = lane; i < ln; i += CBM_WAVE) d[i] = sl[i];
+                        } else {
+                            for (int i = lane; i < ln; i += CBM_WAVE) d[i] = sl[i % dl];
+                        }
+                    } else {  // partly before the span (in dst), partly in it
+                        for (int i = lane; i < ln; i += CBM_WAVE) {
+                            const int q = src + (dl >= ln ? i : i % dl);
+                            d[i] = q < lo ? out[q] : sp[shift + q - lo];
+                        }
+                    }
+                }
+            }
+            res_wave_fence();  // the bytes land before their bits clear
+#pragma unroll
+            for (int u = 0; u < MPL; u++)
+                if (rdy & (1u << u)) mark_bits(pend, shift + dest[u] - lo, len[u], false);
+            res_wave_fence();
+            left &= ~rdy;
+        }
+        // the span back to dst: single bytes up to a 16-byte boundary, 16-byte pieces, single bytes
+        {
+            const int head = ((16 - shift) & 15) < nspan ? ((16 - shift) & 15) : nspan;
+            if (lane < head) out[lo + lane] = sp[shift + lane];
+            const int body = (nspan - head) >> 4;
+            for (int q = lane; q < body; q += CBM_WAVE)
+                *(uint4*)(out + lo + head + 16 * q) = *(const uint4*)(sp + shift + head + 16 * q);
+            for (int i = head + 16 * body + lane; i < nspan; i += CBM_WAVE) out[lo + i] = sp[shift + i];
+        }
+        res_fence();  // later windows read these bytes from dst
+        k0 += c;
+    }
+    if (stuck) {
+        if (lane == 0) *status_b = BLOCK_DEFER;
+        return;
+    }
+    // CRC32: lane j takes the j-th of 64 stretches of 1,024 bytes counted back from the end
+    const int e = L - (CBM_WAVE - 1 - lane) * 1024, st = e - 1024 > 0 ? e - 1024 : 0;
+    uint32_t cr = e > st ? crc_mulmod(xk, crc_bytes16(out + st, e - st, T)) : 0;
+    for (int d = 32; d; d >>= 1) cr ^= __shfl_xor(cr, d, CBM_WAVE);
+    if (lane == 0) {
+        *crc_out_b = cr;
+        if (cr != expect) {
+            *status_b = CBM_BLOCK_CRC;
+            atomicAdd((unsigned long long*)bad, 1ull);
+        }
+    }
+}
+
 template <int MPL, int SPAN, int WPG>
 __global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve(uint8_t* dst, const int64_t* out_off,
                                                                   const int32_t* isize, const uint32_t* expect,
@@ -1479,131 +1630,9 @@ __global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve(uint8_t* dst, 
     uint32_t* pend = s.pend[wv];
     for (int64_t b = (int64_t)blockIdx.x * WPG + wv; b < n_blocks; b += (int64_t)gridDim.x * WPG) {
         if (status[b] != CBM_BLOCK_OK) continue;  // left to the one-phase kernel
-        const int M = mcount[b];
-        uint8_t* out = dst + out_off[b];
-        const uint64_t* ml = pool + (size_t)b * MCAP_TOK;
-        bool stuck = false;
-        for (int k0 = 0; k0 < M && !stuck;) {
-            int dest[MPL], dist[MPL], len[MPL];
-#pragma unroll
-            for (int u = 0; u < MPL; u++) {
-                const int k = k0 + u * CBM_WAVE + lane;
-                const uint64_t m = k < M ? ml[k] : 0;
-                dest[u] = (int)(m & 0x1ffff), dist[u] = (int)((m >> 17) & 0xffff), len[u] = (int)((m >> 33) & 0x1ff);
-            }
-            const int lo = __builtin_amdgcn_readlane(dest[0], 0);
-            // dest + len grows with the match index (outputs are disjoint and in order): the
-            // matches that fit are a prefix, c of them (match k0's at most 258 bytes always fits)
-            int c = 0;
-#pragma unroll
-            for (int u = 0; u < MPL; u++)
-                c += __popcll(__ballot(k0 + u * CBM_WAVE + lane < M && dest[u] + len[u] - lo <= SPAN));
-            int hi = 0;
-#pragma unroll
-            for (int u = 0; u < MPL; u++)
-                if ((c - 1) / CBM_WAVE == u) hi = __builtin_amdgcn_readlane(dest[u] + len[u], (c - 1) % CBM_WAVE);
-            const int nspan = hi - lo;
-            const int shift = (int)((uintptr_t)(out + lo) & 15);
-            // stage the span: the 16-byte aligned pieces that cover it
-            const uint8_t* g0 = out + lo - shift;
-            const int npiece = (shift + nspan + 15) >> 4;
-            for (int q = lane; q < npiece; q += CBM_WAVE) *(uint4*)(sp + 16 * q) = *(const uint4*)(g0 + 16 * q);
-            for (int w = lane; w <= ((shift + nspan) >> 5); w += CBM_WAVE) pend[w] = 0;
-            bool near[MPL];
-#pragma unroll
-            for (int u = 0; u < MPL; u++) {
-                const bool mine = u * CBM_WAVE + lane < c;
-                const int src = dest[u] - dist[u];
-                // far: needs nothing of the window, so its lane copies it at once
-                const bool far = mine && dist[u] >= len[u] && src + len[u] <= lo;
-                if (far) res_copy_far(out + src, sp + shift + dest[u] - lo, len[u]);
-                near[u] = mine && !far;
-            }
-            res_wave_fence();
-#pragma unroll
-            for (int u = 0; u < MPL; u++)
-                if (near[u]) mark_bits(pend, shift + dest[u] - lo, len[u], true);
-            res_wave_fence();
-            uint32_t left = 0;
-#pragma unroll
-            for (int u = 0; u < MPL; u++) left |= near[u] ? 1u << u : 0u;
-            int idle = 0;
-            while (__ballot(left != 0)) {
-                uint32_t rdy = 0;
-#pragma unroll
-                for (int u = 0; u < MPL; u++) {
-                    if (!(left & (1u << u))) continue;
-                    const int src = dest[u] - dist[u], e = src + (dist[u] < len[u] ? dist[u] : len[u]);
-                    const int a0 = src < lo ? lo : src;
-                    if (!(a0 < e && any_bits(pend, shift + a0 - lo, shift + e - lo))) rdy |= 1u << u;
-                }
-                if (!__ballot(rdy != 0) && ++idle > (1 << 12)) {  // cannot happen (the lowest is ready):
-                    stuck = true;                                  // leave the block to the one-phase kernel
-                    break;
-                }
-                // the ready matches, one at a time, each copied by the whole wavefront (a byte a
-                // lane): their destinations are disjoint and pending, their sources final, so they
-                // go in any order. Byte i of a match reads src + i, or src + i mod dist when it
-                // overlaps its own output (a repeat of the dist final bytes before dest).
-#pragma unroll
-                for (int u = 0; u < MPL; u++) {
-                    for (uint64_t rm = __ballot((rdy >> u) & 1u); rm; rm &= rm - 1) {
-                        const int l = __builtin_ctzll(rm);
-                        const int md = __builtin_amdgcn_readlane(dest[u], l);
-                        const int dl = __builtin_amdgcn_readlane(dist[u], l);
-                        const int ln = __builtin_amdgcn_readlane(len[u], l);
-                        const int src = md - dl;
-                        uint8_t* d = sp + shift + md - lo;
-                        if (src >= lo) {  // the source is in the span: LDS only
-                            const uint8_t* sl = sp + shift + src - lo;
-                            if (dl >= ln) {
-                                for (int i = lane; i < ln; i += CBM_WAVE) d[i] = sl[i];
-                            } else {
-                                for (int i = lane; i < ln; i += CBM_WAVE) d[i] = sl[i % dl];
-                            }
-                        } else {  // partly before the span (in dst), partly in it
-                            for (int i = lane; i < ln; i += CBM_WAVE) {
-                                const int q = src + (dl >= ln ? i : i % dl);
-                                d[i] = q < lo ? out[q] : sp[shift + q - lo];
-                            }
-                        }
-                    }
-                }
-                res_wave_fence();  // the bytes land before their bits clear
-#pragma unroll
-                for (int u = 0; u < MPL; u++)
-                    if (rdy & (1u << u)) mark_bits(pend, shift + dest[u] - lo, len[u], false);
-                res_wave_fence();
-                left &= ~rdy;
-            }
-            // the span back to dst: single bytes up to a 16-byte boundary, 16-byte pieces, single bytes
-            {
-                const int head = ((16 - shift) & 15) < nspan ? ((16 - shift) & 15) : nspan;
-                if (lane < head) out[lo + lane] = sp[shift + lane];
-                const int body = (nspan - head) >> 4;
-                for (int q = lane; q < body; q += CBM_WAVE)
-                    *(uint4*)(out + lo + head + 16 * q) = *(const uint4*)(sp + shift + head + 16 * q);
-                for (int i = head + 16 * body + lane; i < nspan; i += CBM_WAVE) out[lo + i] = sp[shift + i];
-            }
-            res_fence();  // later windows read these bytes from dst
-            k0 += c;
-        }
-        if (stuck) {
-            if (lane == 0) status[b] = BLOCK_DEFER;
-            continue;
-        }
-        // CRC32: lane j takes the j-th of 64 stretches of 1,024 bytes counted back from the end
-        const int L = isize[b];
-        const int e = L - (CBM_WAVE - 1 - lane) * 1024, st = e - 1024 > 0 ? e - 1024 : 0;
-        uint32_t cr = e > st ? crc_mulmod(s.crc_x[CBM_WAVE - 1 - lane], crc_bytes16(out + st, e - st, s.crc_t)) : 0;
-        for (int d = 32; d; d >>= 1) cr ^= __shfl_xor(cr, d, CBM_WAVE);
-        if (lane == 0) {
-            crc_out[b] = cr;
-            if (cr != expect[b]) {
-                status[b] = CBM_BLOCK_CRC;
-                atomicAdd((unsigned long long*)bad, 1ull);
-            }
-        }
+        resolve_block<MPL, SPAN>(dst + out_off[b], isize[b], expect[b], crc_out + b, status + b, bad,
+                                 pool + (size_t)b * MCAP_TOK, mcount[b], sp, pend, s.crc_t,
+                                 s.crc_x[CBM_WAVE - 1 - lane], lane);
     }
 }
 

@@ -42,7 +42,7 @@ from cubam_amd import _decode, _lib
 #: matches, or a history ring of 4 to 32 KiB) and resolve blocks per workgroup; then the lane
 #: variant at 1, 2, 4 and 8 blocks per workgroup
 CANDIDATES = tuple(
-    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13)]
+    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14)]
     + [(11, 1, wpg, 0) for wpg in (2, 8)]
     + [(11, 2, wpg, 0) for wpg in (2, 4)]
     + [(11, ring, wpg, 0) for ring, wpg in ((4, 1), (4, 4), (8, 1), (8, 2), (16, 1), (32, 1))]

@@ -104,7 +104,10 @@ enum {
     CBM_INFLATE_TOK64_S512 = 11,
     CBM_INFLATE_TOK64_S1024 = 12,
     CBM_INFLATE_TOK64_S256 = 13,
-    CBM_INFLATE_VARIANTS = 14,
+    /* the two-phase inflate at SEG 512 with each block resolved by the wavefront that decoded it,
+       right after (no resolve launch; blocks_per_wave and waves_per_group are not used) */
+    CBM_INFLATE_TOK64_S512_FUSED = 14,
+    CBM_INFLATE_VARIANTS = 15,
     CBM_INFLATE_DEFAULT = CBM_INFLATE_TOK64_S512 /* a null tune */
 };
 

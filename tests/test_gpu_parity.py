@@ -75,7 +75,7 @@ def test_hetero_blocks_bytes_and_crc(cbm, oracle):
                                                     (11, 0, 0, 0), (11, 0, 0, 5), (12, 0, 0, 0), (13, 0, 0, 0),
                                                     (11, 2, 2, 0), (11, 4, 1, 0), (11, 4, 4, 0), (11, 8, 1, 0), (11, 8, 2, 0), (11, 8, 4, 0),
                                                     (11, 16, 1, 0), (11, 16, 2, 0), (11, 32, 1, 0), (12, 8, 4, 0),
-                                                    (13, 16, 2, 0),
+                                                    (13, 16, 2, 0), (14, 0, 0, 0), (14, 0, 0, 9),
                                                     (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0), (1, 32, 1, 0)])
 def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     from cubam_amd import _lib
@@ -338,7 +338,8 @@ def test_crc32_spans_matches_zlib(cbm):
     assert got == [zlib.crc32(raw[o : o + s].tobytes()) for o, s in zip(off, size)]
 
 
-@pytest.mark.parametrize("variant,bpw,wpg", [(0, 0, 0), (11, 1, 4), (11, 4, 2), (11, 8, 4), (11, 16, 2)])
+@pytest.mark.parametrize("variant,bpw,wpg", [(0, 0, 0), (11, 1, 4), (11, 4, 2), (11, 8, 4), (11, 16, 2),
+                                              (14, 0, 0)])
 @pytest.mark.parametrize("where", ["header", "anywhere"])
 def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant, bpw, wpg):
     """Blocks with a few bits flipped in their dynamic Huffman header (the code-length code and
@@ -395,7 +396,8 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant, bpw, wp
         o += isz[k]
 
 
-@pytest.mark.parametrize("variant,bpw,wpg", [(11, 0, 0), (12, 0, 0), (13, 0, 0), (11, 8, 4), (11, 16, 2)])
+@pytest.mark.parametrize("variant,bpw,wpg", [(11, 0, 0), (12, 0, 0), (13, 0, 0), (11, 8, 4), (11, 16, 2),
+                                              (14, 0, 0)])
 def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant, bpw, wpg):
     """Blocks with more matches than the decode phase's list holds (3-byte matches all through),
     stored blocks, fixed-Huffman blocks, and ordinary ones, mixed: the two-phase inflate leaves the
