@@ -467,6 +467,7 @@ def main():
         step_ms.append((time.perf_counter() - s0) * 1e3)
         inflate_ms.append(ev[0].elapsed_time(ev[1]))
         U, n_rec, K = inf.n, chain.n_rec, column_bytes(chain)
+        n_blocks = int(inf.status.numel())
         del inf, chain, off, cols
         log(f"step {i}: {step_ms[-1]:.2f} ms (inflate {inflate_ms[-1]:.2f} ms)")
     total_s = time.perf_counter() - t0
@@ -497,18 +498,18 @@ def main():
             "p50_step_ms": round(float(np.percentile(step_ms, 50)), 3),
             "p99_step_ms": round(float(np.percentile(step_ms, 99)), 3),
             "path_GBps_C_U_K": round((n + U + K) / (ms / 1e3) / 1e9, 2),
-            "inflate_tune": list(_decode.get_inflate_tune()),
+            "inflate_tune": list(inflate_tune_used(n_blocks, U)),
             "h2d_ms": round(h2d_ms, 3),
             "pcie_inclusive_GBps": round(U / ((ms + h2d_ms) / 1e3) / 1e9, 3),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": _kernel_name(),
+            "kernel": _kernel_name(n_blocks, U),
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": profiled_traffic(_kernel_name()),
+            "traffic": profiled_traffic(_kernel_name(n_blocks, U)),
             "traffic_source": "profiles/hbm_traffic_atac_1GiB_chunk.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
             "algorithmic_bytes": n + U,
             "launch_ms": round(inf_ms, 3),
@@ -555,18 +556,30 @@ def profiled_traffic(shape_label: str):
     return None
 
 
-def _kernel_name() -> str:
+def inflate_tune_used(n_blocks: int, inflated: int) -> tuple[int, int, int, int]:
+    """(variant, blocks_per_wave, waves_per_group, groups) of the inflate of a batch this size: the
+    process's or the autotuned shape, else the library default for the block count."""
     from cubam_amd import _decode
-    from cubam_amd._lib import INFLATE_DEFAULT, INFLATE_SHAPES, INFLATE_TWO_PHASE
+    from cubam_amd._lib import default_variant
 
-    t = _decode.get_inflate_tune()
-    v = t[0] if any(t) else INFLATE_DEFAULT
+    t = _decode.inflate_tune_for(n_blocks, inflated)
+    if t is None:
+        return (default_variant(n_blocks), 0, 0, 0)
+    return (t.variant, t.blocks_per_wave, t.waves_per_group, t.groups)
+
+
+def _kernel_name(n_blocks: int, inflated: int) -> str:
+    from cubam_amd._lib import INFLATE_SHAPES, INFLATE_TOK64_S512_SPLIT, INFLATE_TWO_PHASE
+
+    t = inflate_tune_used(n_blocks, inflated)
+    v = t[0]
     shape = INFLATE_SHAPES[v]
     if shape is None:
         return "cbm_inflate: inflate_lanes + crc_blocks"
     if v in INFLATE_TWO_PHASE:
-        mpl, wpg = t[1] or 1, t[2] or 4
-        return (f"cbm_inflate (two-phase): inflate_wave<64, {shape[1]}, true> + "
+        mpl, wpg = t[1] or 1, t[2] or (2 if v == INFLATE_TOK64_S512_SPLIT else 4)
+        split = " over two streams (blocks cut 3:2)" if v == INFLATE_TOK64_S512_SPLIT else ""
+        return (f"cbm_inflate (two-phase{split}): inflate_wave<64, {shape[1]}, true> + "
                 f"inflate_resolve<{mpl}, {2048 * mpl}, {wpg}> (CRC32 fused in the resolve)")
     return f"cbm_inflate: inflate_wave<{shape[0]}, {shape[1]}> (CRC32 fused)"
 

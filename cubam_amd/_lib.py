@@ -26,15 +26,23 @@ REC_N, REC_TAIL, REC_INVALID, REC_CIGAR, REC_SEQ, REC_QUAL, REC_BAD_LINKS, REC_N
 
 INFLATE_WG512_S128, INFLATE_LANES, INFLATE_WG128, INFLATE_WG512, INFLATE_WG64 = 0, 1, 2, 3, 4
 INFLATE_TOK64_S512, INFLATE_TOK64_S1024, INFLATE_TOK64_S256, INFLATE_TOK64_S512_FUSED = 11, 12, 13, 14
-INFLATE_VARIANTS = 15
+INFLATE_TOK64_S512_SPLIT = 15
+INFLATE_VARIANTS = 16
 #: (threads per block, segment bits) of each workgroup variant; None = one block per lane
 INFLATE_SHAPES = {0: (512, 128), 1: None, 2: (128, 256), 3: (512, 64), 4: (64, 512), 5: (256, 128), 6: (128, 128),
                   7: (64, 128), 8: (64, 256), 9: (384, 128), 10: (256, 256), 11: (64, 512), 12: (64, 1024),
-                  13: (64, 256), 14: (64, 512)}
+                  13: (64, 256), 14: (64, 512), 15: (64, 512)}
 #: the two-phase variants (decode to literals and match lists, then resolve)
-INFLATE_TWO_PHASE = (INFLATE_TOK64_S512, INFLATE_TOK64_S1024, INFLATE_TOK64_S256, INFLATE_TOK64_S512_FUSED)
-#: the variant a null tune selects (CBM_INFLATE_DEFAULT)
+INFLATE_TWO_PHASE = (INFLATE_TOK64_S512, INFLATE_TOK64_S1024, INFLATE_TOK64_S256, INFLATE_TOK64_S512_FUSED,
+                     INFLATE_TOK64_S512_SPLIT)
+#: the variant a null tune selects below INFLATE_SPLIT_MIN blocks (CBM_INFLATE_DEFAULT), and from there on
 INFLATE_DEFAULT = INFLATE_TOK64_S512
+INFLATE_SPLIT_MIN = 4096
+
+
+def default_variant(n_blocks: int) -> int:
+    """The variant a null tune selects for an inflate of ``n_blocks`` blocks."""
+    return INFLATE_TOK64_S512_SPLIT if n_blocks >= INFLATE_SPLIT_MIN else INFLATE_DEFAULT
 
 
 class InflateTune(ctypes.Structure):

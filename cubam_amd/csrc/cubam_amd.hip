@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -256,20 +257,17 @@ static int launch_resolve_ring(uint8_t* dst, const int64_t* out_off, const int32
     return check_launch("inflate_resolve_ring");
 }
 
-template <int SEG, bool FUSE = false>
-static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
-                      const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
-                      uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int res_mpl, int res_wpg,
-                      void* ws, int64_t ws_groups, int64_t max_groups, hipStream_t st) {
-    int64_t* counter = (int64_t*)ws;  // [0] decode-phase next block, [1] one-phase next block, [2] blocks left
-    uint64_t* mlists = (uint64_t*)((uint8_t*)ws + INFLATE_WS_HDR);
-    uint8_t* tok = (uint8_t*)mlists + ws_groups * (int64_t)MLIST_CAP * 8;
-    int64_t* idx = (int64_t*)tok;
-    int32_t* mcount = (int32_t*)(idx + n_blocks);
-    uint64_t* pool = (uint64_t*)(tok + ((12 * n_blocks + 255) / 256) * 256);
+// The decode and resolve phases of blocks [0, n_blocks) of the arrays passed (a part of a call's
+// blocks: the pointers start at the part's first block), on st; next_block counts the decode
+// phase's blocks.
+template <int SEG, bool FUSE>
+static int tok_part(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                    const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                    uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int res_mpl, int res_wpg,
+                    int64_t* next_block, int64_t* stats, uint64_t* pool, int32_t* mcount, hipStream_t st) {
     int rc = launch_inflate_wave<64, SEG, true, FUSE>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst,
-                                                      crc, status, result, groups, (int64_t)1 << 40, counter,
-                                                      nullptr, counter + 8, nullptr, nullptr, pool, mcount, st);
+                                                      crc, status, result, groups, (int64_t)1 << 40, next_block,
+                                                      nullptr, stats, nullptr, nullptr, pool, mcount, st);
     if (rc) return rc;
     switch (FUSE ? 0 : res_mpl * 100 + res_wpg) {
         case 0:  // resolved by the decode launch
@@ -304,7 +302,103 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
             set_error("cbm_inflate: two-phase resolve shape (blocks_per_wave, waves_per_group) not built");
             return CBM_ERR_ARG;
     }
-    if (rc) return rc;
+    return rc;
+}
+
+// A side stream for each (device, calling stream), with the events that fork work onto it and
+// join it back; created on first use and kept for the life of the process. Streams at the
+// default priority.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    std::mutex mu;  // held from the fork to the join, so that calls on one stream pair their events
+};
+static std::mutex g_side_mu;
+static std::map<std::pair<int, hipStream_t>, SideStream> g_side;
+
+// *out = null when there is none yet and `create` is false (no stream is created while st captures
+// a graph: work forked under capture goes to a side stream made by an earlier call on st)
+static int side_stream(hipStream_t st, bool create, SideStream** out) {
+    *out = nullptr;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("cbm_inflate: no current device");
+        return CBM_ERR_LAUNCH;
+    }
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    const auto key = std::make_pair(dev, st);
+    auto it = g_side.find(key);
+    if (it != g_side.end() && it->second.s) {
+        *out = &it->second;
+        return CBM_OK;
+    }
+    if (!create) return CBM_OK;
+    hipStream_t ns = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    if (hipStreamCreateWithFlags(&ns, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
+        if (join) (void)hipEventDestroy(join);
+        if (fork) (void)hipEventDestroy(fork);
+        if (ns) (void)hipStreamDestroy(ns);
+        set_error("cbm_inflate: side stream or event creation failed");
+        return CBM_ERR_LAUNCH;
+    }
+    SideStream& e = g_side[key];
+    e.fork = fork;
+    e.join = join;
+    e.s = ns;
+    *out = &e;
+    return CBM_OK;
+}
+
+// Is st capturing a graph (or is its state unknown)?
+static bool capturing(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+template <int SEG, bool FUSE = false>
+static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t* bsize, const int32_t* isize,
+                      const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
+                      uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int res_mpl, int res_wpg,
+                      void* ws, int64_t ws_groups, int64_t max_groups, bool split, hipStream_t st) {
+    // [0] decode-phase next block (first part), [1] one-phase next block, [2] blocks left,
+    // [3] decode-phase next block of the second part
+    int64_t* counter = (int64_t*)ws;
+    uint64_t* mlists = (uint64_t*)((uint8_t*)ws + INFLATE_WS_HDR);
+    uint8_t* tok = (uint8_t*)mlists + ws_groups * (int64_t)MLIST_CAP * 8;
+    int64_t* idx = (int64_t*)tok;
+    int32_t* mcount = (int32_t*)(idx + n_blocks);
+    uint64_t* pool = (uint64_t*)(tok + ((12 * n_blocks + 255) / 256) * 256);
+    int rc;
+    SideStream* side = nullptr;
+    if (split && !FUSE && n_blocks >= 2 && (rc = side_stream(st, !capturing(st), &side))) return rc;
+    if (!side) {
+        rc = tok_part<SEG, FUSE>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, result,
+                                 groups, res_mpl, res_wpg, counter, counter + 8, pool, mcount, st);
+        if (rc) return rc;
+    } else {
+        const int64_t cut = n_blocks * 3 / 5 > 0 ? n_blocks * 3 / 5 : 1;
+        std::lock_guard<std::mutex> lock(side->mu);
+        if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess) {
+            set_error("cbm_inflate: fork to the side stream failed");
+            return CBM_ERR_LAUNCH;
+        }
+        rc = tok_part<SEG, FUSE>(src, blk_off, bsize, isize, crc_expect, out_off, cut, dst, crc, status, result,
+                                 groups, res_mpl, res_wpg, counter, counter + 8, pool, mcount, st);
+        // the second part's launches go to the side stream even when the first part's failed, so
+        // that the join below always follows them
+        const int rc2 = tok_part<SEG, FUSE>(src, blk_off + cut, bsize + cut, isize + cut, crc_expect + cut,
+                                            out_off + cut, n_blocks - cut, dst, crc + cut, status + cut, result,
+                                            groups, res_mpl, res_wpg, counter + 3, counter + 8,
+                                            pool + cut * (int64_t)MCAP_TOK, mcount + cut, side->s);
+        if (hipEventRecord(side->join, side->s) != hipSuccess || hipStreamWaitEvent(st, side->join, 0) != hipSuccess) {
+            set_error("cbm_inflate: join from the side stream failed");
+            return CBM_ERR_LAUNCH;
+        }
+        if (rc || rc2) return rc ? rc : rc2;
+    }
     int64_t lg = (n_blocks + 255) / 256;
     if (lg > 1024) lg = 1024;
     hipLaunchKernelGGL(inflate_defer_list, dim3((unsigned)lg), dim3(256), 0, st, status, n_blocks, idx, counter + 2);
@@ -430,7 +524,7 @@ int64_t cbm_inflate_workspace(int64_t n_blocks) {
 
 }  // extern "C"
 
-static bool tok_variant(int v) { return v >= CBM_INFLATE_TOK64_S512 && v <= CBM_INFLATE_TOK64_S512_FUSED; }
+static bool tok_variant(int v) { return v >= CBM_INFLATE_TOK64_S512 && v <= CBM_INFLATE_TOK64_S512_SPLIT; }
 
 // cbm_inflate's work once the workspace is settled: ws holds INFLATE_WS_HDR bytes and max_groups
 // match lists (unused by the lane variant), and with `tok` the two-phase inflate's tok_ws_bytes
@@ -443,13 +537,17 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
     if (rc || n_blocks == 0) return rc;
     ARG(src && blk_off && bsize && isize && crc_expect && out_off && dst && crc && status,
         "cbm_inflate: null pointer");
-    int variant = tune ? tune->variant : CBM_INFLATE_DEFAULT;
+    int variant = tune ? tune->variant
+                       : n_blocks >= CBM_INFLATE_SPLIT_MIN ? CBM_INFLATE_TOK64_S512_SPLIT : CBM_INFLATE_DEFAULT;
     if (tok_variant(variant) && !tok) variant = CBM_INFLATE_WG512_S128;
     const int64_t groups = tune ? tune->groups : 0;
     // two-phase variants: blocks_per_wave = matches a lane of the resolve phase, waves_per_group =
     // its blocks a workgroup
     const int res_mpl = tune && tune->blocks_per_wave > 0 ? tune->blocks_per_wave : 1;
-    const int res_wpg = tune && tune->waves_per_group > 0 ? tune->waves_per_group : 4;
+    // (two blocks a workgroup with the split: its parts' resolves share the CUs with decode launches)
+    const int res_wpg = tune && tune->waves_per_group > 0 ? tune->waves_per_group
+                        : variant == CBM_INFLATE_TOK64_S512_SPLIT ? 2
+                                                                  : 4;
     if (variant != CBM_INFLATE_LANES) {
         int64_t* counter = (int64_t*)ws;  // [0] next block, [8..] decoder statistics
         if ((rc = fill(counter, 8 + STAT_N, 0, st))) return rc;
@@ -472,7 +570,7 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
 #define TOK_CASE(V, SEG_)                                                                                    \
     case V:                                                                                                  \
         return launch_tok<SEG_>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status, \
-                                result, groups, res_mpl, res_wpg, ws, inflate_max_groups(), max_groups, st);
+                                result, groups, res_mpl, res_wpg, ws, inflate_max_groups(), max_groups, false, st);
             TOK_CASE(CBM_INFLATE_TOK64_S512, 512)
             TOK_CASE(CBM_INFLATE_TOK64_S1024, 1024)
             TOK_CASE(CBM_INFLATE_TOK64_S256, 256)
@@ -480,7 +578,11 @@ static int inflate_impl(const uint8_t* src, const int64_t* blk_off, const int32_
             case CBM_INFLATE_TOK64_S512_FUSED:
                 return launch_tok<512, true>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
                                              status, result, groups, res_mpl, res_wpg, ws, inflate_max_groups(),
-                                             max_groups, st);
+                                             max_groups, false, st);
+            case CBM_INFLATE_TOK64_S512_SPLIT:
+                return launch_tok<512>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc, status,
+                                       result, groups, res_mpl, res_wpg, ws, inflate_max_groups(), max_groups, true,
+                                       st);
             default: ARG(false, "cbm_inflate: unknown variant");
         }
     }

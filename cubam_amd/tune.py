@@ -42,11 +42,12 @@ from cubam_amd import _decode, _lib
 #: matches, or a history ring of 4 to 32 KiB) and resolve blocks per workgroup; then the lane
 #: variant at 1, 2, 4 and 8 blocks per workgroup
 CANDIDATES = tuple(
-    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14)]
+    [(v, 0, 0, 0) for v in (0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15)]
     + [(11, 1, wpg, 0) for wpg in (2, 8)]
     + [(11, 2, wpg, 0) for wpg in (2, 4)]
     + [(11, ring, wpg, 0) for ring, wpg in ((4, 1), (4, 4), (8, 1), (8, 2), (16, 1), (32, 1))]
     + [(v, 1, 2, 0) for v in (12, 13)]
+    + [(15, 1, wpg, 0) for wpg in (4, 8)]
     + [(_lib.INFLATE_LANES, bpw, 1, 0) for bpw in (1, 2, 4, 8)]
     + [(_lib.INFLATE_LANES, 4, 2, 0), (_lib.INFLATE_LANES, 8, 2, 0)]
 )
@@ -170,7 +171,8 @@ class GraphInflate:
     The compressed buffer, the block table and the outputs stay at fixed device
     addresses: write new compressed bytes of the same block layout into
     ``blocks.buf`` and call :meth:`run`. The graph holds only kernels (the
-    workspace counters are cleared by a fill kernel, not a memset).
+    workspace counters are cleared by a fill kernel, not a memset), and with the
+    two-stream inflate the fork to its side stream and the join back.
     """
 
     def __init__(self, blocks: _decode.Blocks, tune=None, ws: _decode.Workspace | None = None):
@@ -185,10 +187,15 @@ class GraphInflate:
         self.status = torch.empty(max(nb, 1), dtype=torch.int32, device=dev)
         self.bad = torch.empty(1, dtype=torch.int64, device=dev)
         self.ws = torch.empty(L.cbm_inflate_workspace(nb), dtype=torch.uint8, device=dev)
-        self._launch()  # eager once: kernel attributes are set outside the capture
+        # eager once on the stream the capture uses: kernel attributes are set, and the side stream a
+        # two-stream inflate forks to is made, outside the capture
+        self.stream = torch.cuda.Stream()
+        self.stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self.stream):
+            self._launch()
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
+        with torch.cuda.graph(self.graph, stream=self.stream):
             self._launch()
         torch.cuda.synchronize()
 

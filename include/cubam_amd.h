@@ -85,7 +85,8 @@ enum {
 
 /* Inflate kernel variants (cbm_inflate_tune.variant): a workgroup of NT threads decodes each
    block, each thread SEG bits of a round; or one block per lane (CBM_INFLATE_LANES); or the
-   two-phase variants. A null tune selects CBM_INFLATE_DEFAULT. */
+   two-phase variants. A null tune selects CBM_INFLATE_TOK64_S512_SPLIT for CBM_INFLATE_SPLIT_MIN
+   blocks or more, else CBM_INFLATE_TOK64_S512 (CBM_INFLATE_DEFAULT). */
 enum {
     CBM_INFLATE_WG512_S128 = 0, /* NT 512, SEG 128: the one-phase kernel the others fall back to */
     CBM_INFLATE_LANES = 1,      /* blocks_per_wave lanes of a wavefront each inflate a block */
@@ -107,9 +108,19 @@ enum {
     /* the two-phase inflate at SEG 512 with each block resolved by the wavefront that decoded it,
        right after (no resolve launch; blocks_per_wave and waves_per_group are not used) */
     CBM_INFLATE_TOK64_S512_FUSED = 14,
-    CBM_INFLATE_VARIANTS = 15,
-    CBM_INFLATE_DEFAULT = CBM_INFLATE_TOK64_S512 /* a null tune */
+    /* CBM_INFLATE_TOK64_S512 with the blocks cut in two at 3/5: the first part decodes and resolves
+       on `stream`, the second at the same time on a side stream of the library's (one per device
+       and calling stream, default priority, kept for the life of the process), joined back into
+       `stream` before the blocks the first phase left are decoded; a part's resolve overlaps the
+       other's decode and the tails of the persistent launches. Under graph capture the fork and
+       join are captured too, onto the side stream an earlier call on the same stream made; with
+       none, it runs as CBM_INFLATE_TOK64_S512 on `stream` alone. */
+    CBM_INFLATE_TOK64_S512_SPLIT = 15,
+    CBM_INFLATE_VARIANTS = 16,
+    CBM_INFLATE_DEFAULT = CBM_INFLATE_TOK64_S512 /* a null tune below CBM_INFLATE_SPLIT_MIN blocks */
 };
+/* blocks from which a null tune splits the two-phase inflate over two streams */
+#define CBM_INFLATE_SPLIT_MIN 4096
 
 /* Launch shape of the inflate kernel; zero fields select the variant's defaults. */
 typedef struct {
@@ -121,9 +132,10 @@ typedef struct {
     int32_t blocks_per_wave;
     /* CBM_INFLATE_LANES: wavefronts per workgroup, 1..16 (default 2); two-phase variants: blocks
        (one a wavefront) per workgroup of the resolve pass: 2, 4 or 8 with 1 match a lane (default
-       4), 2 or 4 with 2; with a ring, 1, 2 or 4 (4 KiB, 8 KiB), 1 or 2 (16 KiB), 1 (32 KiB) */
+       4, and 2 for CBM_INFLATE_TOK64_S512_SPLIT), 2 or 4 with 2; with a ring, 1, 2 or 4 (4 KiB,
+       8 KiB), 1 or 2 (16 KiB), 1 (32 KiB) */
     int32_t waves_per_group;
-    int32_t groups; /* workgroups of the (first) decode launch; 0 = as many as the CUs hold */
+    int32_t groups; /* workgroups of each decode launch; 0 = as many as the CUs hold */
 } cbm_inflate_tune;
 
 /* Bytes of workspace cbm_inflate needs for n_blocks blocks. */

@@ -64,7 +64,7 @@ def test_autotune_picks_and_caches(hetero, tmp_path, monkeypatch):
         monkeypatch.setattr(tune, "_cache", None)
 
 
-@pytest.mark.parametrize("shape", [(0, 0, 0, 0), (1, 8, 1, 0)])
+@pytest.mark.parametrize("shape", [(0, 0, 0, 0), (1, 8, 1, 0), (15, 0, 0, 0)])
 def test_graph_replays_match_eager(hetero, shape):
     """Every replay of the captured launch gives the eager result, with other work in between."""
     import torch

@@ -76,6 +76,7 @@ def test_hetero_blocks_bytes_and_crc(cbm, oracle):
                                                     (11, 2, 2, 0), (11, 4, 1, 0), (11, 4, 4, 0), (11, 8, 1, 0), (11, 8, 2, 0), (11, 8, 4, 0),
                                                     (11, 16, 1, 0), (11, 16, 2, 0), (11, 32, 1, 0), (12, 8, 4, 0),
                                                     (13, 16, 2, 0), (14, 0, 0, 0), (14, 0, 0, 9),
+                                                    (15, 0, 0, 0), (15, 0, 0, 5), (15, 8, 4, 0),
                                                     (1, 1, 1, 0), (1, 4, 2, 0), (1, 16, 3, 0), (1, 32, 1, 0)])
 def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     from cubam_amd import _lib
@@ -86,6 +87,36 @@ def test_inflate_every_launch_shape(cbm, variant, bpw, wpg, groups):
     blocks, inf = _gpu_blocks(hb.bgzf, tune)
     assert int(inf.result.item()) == 0
     assert inf.data[: inf.n].cpu().numpy().tobytes() == hb.raw.tobytes()
+
+
+def test_split_inflates_on_two_streams_at_once(cbm):
+    """The two-stream inflate called on two streams together (each forks its own side stream):
+    each result is the one a single-stream inflate of the same blocks gives."""
+    import torch
+
+    from cubam_amd import _decode, _lib
+
+    hbs = [_hetero(500, 5), _hetero(650, 6)]
+    ref = []
+    for hb in hbs:
+        _, inf = _gpu_blocks(hb.bgzf, _lib.InflateTune(11, 0, 0, 0))
+        ref.append((inf.data[: inf.n].cpu().numpy().tobytes(), inf.crc.cpu().numpy().tolist()))
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    bufs = [_decode.to_device(hb.bgzf) for hb in hbs]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        outs = []
+        for hb, st, buf in zip(hbs, streams, bufs):
+            with torch.cuda.stream(st):
+                ws = _decode.Workspace()
+                blocks = _decode.scan_blocks(buf, int(hb.bgzf.size), True, ws)
+                outs.append(_decode.inflate_blocks(blocks, None, ws, tune=_lib.InflateTune(15, 0, 0, 0)))
+        torch.cuda.synchronize()
+        for (data, crc), inf in zip(ref, outs):
+            assert int(inf.result.item()) == 0
+            assert inf.data[: inf.n].cpu().numpy().tobytes() == data
+            assert inf.crc.cpu().numpy().tolist() == crc
+        _say("two streams, rep", rep, "ok")
 
 
 def test_tiny_and_empty_blocks(cbm):
@@ -339,7 +370,7 @@ def test_crc32_spans_matches_zlib(cbm):
 
 
 @pytest.mark.parametrize("variant,bpw,wpg", [(0, 0, 0), (11, 1, 4), (11, 4, 2), (11, 8, 4), (11, 16, 2),
-                                              (14, 0, 0)])
+                                              (14, 0, 0), (15, 0, 0)])
 @pytest.mark.parametrize("where", ["header", "anywhere"])
 def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant, bpw, wpg):
     """Blocks with a few bits flipped in their dynamic Huffman header (the code-length code and
@@ -397,7 +428,7 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant, bpw, wp
 
 
 @pytest.mark.parametrize("variant,bpw,wpg", [(11, 0, 0), (12, 0, 0), (13, 0, 0), (11, 8, 4), (11, 16, 2),
-                                              (14, 0, 0)])
+                                              (14, 0, 0), (15, 0, 0)])
 def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant, bpw, wpg):
     """Blocks with more matches than the decode phase's list holds (3-byte matches all through),
     stored blocks, fixed-Huffman blocks, and ordinary ones, mixed: the two-phase inflate leaves the
