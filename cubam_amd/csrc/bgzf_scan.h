@@ -67,29 +67,65 @@ __device__ __forceinline__ uint64_t thread_candidates(const uint8_t* b, int64_t 
     return mask;
 }
 
-__global__ void __launch_bounds__(BGZF_NT) bgzf_count(const uint8_t* b, int64_t n, int64_t* tile_cnt) {
+// Block starts a tile keeps for bgzf_emit (a 16 KiB tile of blocks of average size holds one or
+// two); a tile with more is scanned again there
+constexpr int BGZF_SLOTS = 8;
+
+// Candidates of each tile: their count to tile_cnt, and themselves, in order, to the tile's slots
+// when they fit there
+__global__ void __launch_bounds__(BGZF_NT) bgzf_count(const uint8_t* b, int64_t n, int64_t* tile_cnt,
+                                                      int64_t* slots) {
     __shared__ int64_t sh[BGZF_NT / CBM_WAVE];
     int64_t o0 = (int64_t)blockIdx.x * BGZF_TILE + (int64_t)threadIdx.x * 64;
     uint64_t m = thread_candidates(b, n, o0);
     int64_t tot;
-    block_exclusive_sum<BGZF_NT, int64_t>((int64_t)__popcll(m), sh, &tot);
+    int64_t k = block_exclusive_sum<BGZF_NT, int64_t>((int64_t)__popcll(m), sh, &tot);
+    if (tot <= BGZF_SLOTS) {
+        int64_t* slot = slots + (int64_t)blockIdx.x * BGZF_SLOTS;
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            slot[k++] = o0 + j;
+        }
+    }
     if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
 }
 
+// The candidates in order at cand[tile_base[t] ..], a thread per tile: copied from the tile's slots,
+// or (a tile with more than its slots hold) scanned again by the whole workgroup. *total is the
+// number of candidates (the base after the last tile).
 __global__ void __launch_bounds__(BGZF_NT) bgzf_emit(const uint8_t* b, int64_t n, const int64_t* tile_base,
+                                                     const int64_t* total, int64_t tiles, const int64_t* slots,
                                                      int64_t cap, int64_t* cand) {
     __shared__ int64_t sh[BGZF_NT / CBM_WAVE];
-    int64_t t = blockIdx.x;
-    if (tile_base[t + 1] == tile_base[t]) return;  // no candidate in this tile (uniform per block)
-    int64_t o0 = t * BGZF_TILE + (int64_t)threadIdx.x * 64;
-    uint64_t m = thread_candidates(b, n, o0);
-    int64_t tot;
-    int64_t k = tile_base[t] + block_exclusive_sum<BGZF_NT, int64_t>((int64_t)__popcll(m), sh, &tot);
-    while (m) {
-        int j = __builtin_ctzll(m);
-        m &= m - 1;
-        if (k < cap) cand[k] = o0 + j;
-        k++;
+    __shared__ int64_t over[BGZF_NT];
+    __shared__ int n_over;
+    if (threadIdx.x == 0) n_over = 0;
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * BGZF_NT + threadIdx.x;
+    if (t < tiles) {
+        const int64_t k0 = tile_base[t], c = (t + 1 < tiles ? tile_base[t + 1] : *total) - k0;
+        if (c > BGZF_SLOTS) {
+            over[atomicAdd(&n_over, 1)] = t;
+        } else {
+            for (int64_t j = 0; j < c && k0 + j < cap; j++) cand[k0 + j] = slots[t * BGZF_SLOTS + j];
+        }
+    }
+    __syncthreads();
+    const int no = n_over;
+    for (int i = 0; i < no; i++) {
+        const int64_t u = over[i];
+        const int64_t o0 = u * BGZF_TILE + (int64_t)threadIdx.x * 64;
+        uint64_t m = thread_candidates(b, n, o0);
+        int64_t tot;
+        int64_t k = tile_base[u] + block_exclusive_sum<BGZF_NT, int64_t>((int64_t)__popcll(m), sh, &tot);
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            if (k < cap) cand[k] = o0 + j;
+            k++;
+        }
+        __syncthreads();  // sh is read by every thread before the next tile's sum writes it
     }
 }
 

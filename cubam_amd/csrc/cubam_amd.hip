@@ -469,7 +469,7 @@ static int64_t bgzf_max_blocks(int64_t n) { return n / 26 + 2; }
 int64_t cbm_bgzf_scan_workspace(int64_t n) {
     int64_t tiles = (n + BGZF_TILE - 1) / BGZF_TILE + 1;
     int64_t m = bgzf_max_blocks(n);
-    return 8 * (8 + tiles + 2) + scan_workspace(m > tiles ? m : tiles) + 64;
+    return 8 * (8 + tiles + 2) + 8 * BGZF_SLOTS * tiles + scan_workspace(m > tiles ? m : tiles) + 64;
 }
 
 int cbm_bgzf_scan(const uint8_t* buf, int64_t n, int eof, int64_t cap, int64_t* blk_off, int32_t* bsize,
@@ -484,15 +484,17 @@ int cbm_bgzf_scan(const uint8_t* buf, int64_t n, int eof, int64_t cap, int64_t* 
     int64_t* hdr = (int64_t*)ws;
     int64_t tiles = (n + BGZF_TILE - 1) / BGZF_TILE;
     int64_t* tcnt = hdr + 8;
-    void* sws = tcnt + tiles + 2;
+    int64_t* slots = tcnt + tiles + 2;
+    void* sws = slots + BGZF_SLOTS * tiles;
     int rc = fill(hdr, 8, 0, st);
     if (rc) return rc;
     if (tiles > 0) {
-        hipLaunchKernelGGL(bgzf_count, dim3((unsigned)tiles), dim3(BGZF_NT), 0, st, buf, n, tcnt);
+        hipLaunchKernelGGL(bgzf_count, dim3((unsigned)tiles), dim3(BGZF_NT), 0, st, buf, n, tcnt, slots);
         if ((rc = check_launch("bgzf_count"))) return rc;
         if ((rc = exclusive_scan(LoadI64{tcnt}, StoreI64{tcnt}, (const int64_t*)nullptr, tiles, sws, hdr, st)))
             return rc;
-        hipLaunchKernelGGL(bgzf_emit, dim3((unsigned)tiles), dim3(BGZF_NT), 0, st, buf, n, tcnt, cap, blk_off);
+        hipLaunchKernelGGL(bgzf_emit, dim3((unsigned)((tiles + BGZF_NT - 1) / BGZF_NT)), dim3(BGZF_NT), 0, st, buf, n,
+                           tcnt, hdr, tiles, slots, cap, blk_off);
         if ((rc = check_launch("bgzf_emit"))) return rc;
         int64_t g = (cap + 255) / 256;
         if (g > 1024) g = 1024;
