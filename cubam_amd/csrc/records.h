@@ -454,7 +454,21 @@ __global__ void __launch_bounds__(RW_WAVES * CBM_WAVE)
     const int64_t lim = have ? (n - abase < RW_STAGE ? n - abase : RW_STAGE) : 0;
     uint8_t* l = stg[wv];
     const int nvec = vec ? (int)(lim / 16) : 0;
-    for (int i = lane; i < nvec; i += CBM_WAVE) ((uint4*)l)[i] = *(const uint4*)(b + abase + 16LL * i);
+    // the vectors a lane stages, loaded four at a time before any is written (a vector past the
+    // window loads the lane's first one again), so that the loads are in flight together
+    for (int i0 = lane; i0 < nvec; i0 += 4 * CBM_WAVE) {
+        uint4 t[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * CBM_WAVE;
+            t[u] = *(const uint4*)(b + abase + 16LL * (i < nvec ? i : i0));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * CBM_WAVE;
+            if (i < nvec) ((uint4*)l)[i] = t[u];
+        }
+    }
     for (int i = nvec * 16 + lane; i < lim; i += CBM_WAVE) l[i] = b[abase + i];
     __syncthreads();
     if (!have) return;
@@ -649,10 +663,12 @@ __device__ __forceinline__ void stage_records(uint8_t* st, const uint8_t* b, int
     constexpr int SB = 4;
     for (int64_t k0 = (int64_t)threadIdx.x * 16; k0 < nb; k0 += (int64_t)FIELDS_THREADS * 16 * SB) {
         uint4 t[SB];
+        // every load issued, unconditionally (a vector past the range loads the thread's first one
+        // again), so that the SB loads are in flight together and t stays in registers
 #pragma unroll
         for (int u = 0; u < SB; u++) {
             const int64_t k = k0 + (int64_t)u * FIELDS_THREADS * 16;
-            if (k < nb) t[u] = *(const uint4*)(b + a0 + k);
+            t[u] = *(const uint4*)(b + a0 + (k < nb ? k : k0));
         }
 #pragma unroll
         for (int u = 0; u < SB; u++) {
