@@ -706,10 +706,10 @@ __device__ __forceinline__ int csr_record(const int64_t* sh_off, int m, int64_t 
 }
 
 // CIGAR words: one thread per word
-__device__ __forceinline__ void csr_fill_words(uint32_t* out, const int64_t* ooff, int64_t i0, int m,
+__device__ __forceinline__ void csr_fill_words(uint32_t* out, int64_t base, int64_t end, int m,
                                                const int64_t* sh_off, const int64_t* sh_src, const uint8_t* st,
                                                int64_t a0, int64_t hi, const uint8_t* b) {
-    const int64_t base = ooff[i0], total = ooff[i0 + m] - base;
+    const int64_t total = end - base;
     const CBM_LDS uint32_t* sw = (const CBM_LDS uint32_t*)st;
     const CBM_GLOBAL uint8_t* bg = (const CBM_GLOBAL uint8_t*)b;
     for (int64_t j = threadIdx.x; j < total; j += FIELDS_THREADS) {
@@ -730,10 +730,9 @@ __device__ __forceinline__ void csr_fill_words(uint32_t* out, const int64_t* oof
 // Bases or qualities: each thread fills one 16-byte aligned piece of the output (one search,
 // then a walk over the records it spans), stored with one 16-byte store when the piece lies
 // wholly in this workgroup's range.
-__device__ __forceinline__ void csr_fill_bytes(uint8_t* out, const int64_t* ooff, int64_t i0, int m,
+__device__ __forceinline__ void csr_fill_bytes(uint8_t* out, int64_t base, int64_t end, int m,
                                                const int64_t* sh_off, const int64_t* sh_src, const uint8_t* st,
                                                int64_t a0, int64_t hi, const uint8_t* b) {
-    const int64_t base = ooff[i0], end = ooff[i0 + m];
     if (end <= base) return;
     const uintptr_t ob = (uintptr_t)out;
     // pieces: [p, p + 16) with out + p 16-byte aligned
@@ -777,35 +776,56 @@ __global__ void __launch_bounds__(FIELDS_THREADS) rec_columns(const uint8_t* b, 
                                                          cbm_columns c) {
     __shared__ __align__(16) uint8_t st[SC_STAGE + 16];
     __shared__ int64_t sh_off[FIELDS_NT + 1], sh_src[FIELDS_NT];
-    __shared__ int64_t range[2];
+    // [0] stage start, [1] stage end, [2] the range's first offset, [3] the offset after the
+    // workgroup's records (the next record's, or the last record's end), [4..6] / [7..9] the
+    // CIGAR, base and quality CSR offsets of the first record / after the last
+    __shared__ int64_t range[10];
     const int64_t i0 = (int64_t)blockIdx.x * FIELDS_NT;
     if (i0 >= n) return;
     const int m = (int)(n - i0 < FIELDS_NT ? n - i0 : FIELDS_NT);
-    if (threadIdx.x == 0) {
-        const int64_t lo = off[i0], last = off[i0 + m - 1];
-        const int64_t hi = last + 4 + (int64_t)ld_u32(b, last);
+    const int t = threadIdx.x;
+    // every load of the workgroup's offsets at once: the record offsets and the three CSR columns'
+    // offsets of records i0 .. i0 + m (CSR columns hold n + 1 entries)
+    const bool cig_on = c.cigar && c.cigar_off, seq_on = c.seq && c.seq_off, qual_on = c.qual && c.qual_off;
+    int64_t o = 0, co = 0, so = 0, qo = 0;
+    if (t <= m) {
+        const int64_t i = i0 + t;
+        if (i < n) o = off[i];
+        if (cig_on) co = c.cigar_off[i];
+        if (seq_on) so = c.seq_off[i];
+        if (qual_on) qo = c.qual_off[i];
+        if (t == 0) range[2] = o, range[4] = co, range[5] = so, range[6] = qo;
+        if (t == m) range[7] = co, range[8] = so, range[9] = qo;
+        // the records of a chain are back to back: the next record's offset ends the range; the
+        // last workgroup reads the last record's size
+        if (t == m && i < n) range[3] = o;
+        if (t == m - 1 && i0 + m == n) range[3] = o + 4 + (int64_t)ld_u32(b, o);
+    }
+    __syncthreads();
+    if (t == 0) {
+        const int64_t lo = range[2], end = range[3];
         const int64_t a0 = lo - (int64_t)(((uintptr_t)(b + lo)) & 15);
         range[0] = a0;
-        // as much of the range as the stage holds; records past it are read in place
-        range[1] = last >= lo ? (hi - a0 <= SC_STAGE ? hi : a0 + SC_STAGE) : a0;
+        // as much of the range as the stage holds; records past it (or outside it, for offsets
+        // that are not in order) are read in place
+        range[1] = end >= lo ? (end - a0 <= SC_STAGE ? end : a0 + SC_STAGE) : a0;
     }
     __syncthreads();
     const int64_t a0 = range[0], hi = range[1];
     stage_records(st, b, a0, hi - a0);
     __syncthreads();
     // fixed fields and tags (decode_fixed and find_tag)
-    if (threadIdx.x < m) {
-        const int64_t i = i0 + threadIdx.x, o = off[i];
+    if (t < m) {
+        const int64_t i = i0 + t;
         bool staged = o >= a0 && o + 36 <= hi;
         if (staged) staged = o + 4 + (int64_t)ld_u32(st, o - a0) <= hi;
         if (staged) parse_record<true>(st, o - a0, a0, i, c, nullptr, nullptr);
         else parse_record<false>(b, o, 0, i, c, nullptr, nullptr);
     }
-    if (!(c.cigar && c.cigar_off) && !(c.seq && c.seq_off) && !(c.qual && c.qual_off)) return;
+    if (!cig_on && !seq_on && !qual_on) return;
     // where each record's CIGAR, bases and qualities start (data offsets)
     int64_t cig = 0, sq = 0, ql = 0;
-    if (threadIdx.x < m) {
-        const int64_t o = off[i0 + threadIdx.x];
+    if (t < m) {
         const bool staged = o >= a0 && o + 36 <= hi;
         const uint8_t* p = staged ? st : b;
         const int64_t r = staged ? o - a0 : o;
@@ -814,31 +834,31 @@ __global__ void __launch_bounds__(FIELDS_THREADS) rec_columns(const uint8_t* b, 
         sq = cig + 4LL * ncig;
         ql = sq + (lseq + 1) / 2;
     }
-    if (c.cigar && c.cigar_off) {
-        if (threadIdx.x < m) {
-            sh_off[threadIdx.x] = c.cigar_off[i0 + threadIdx.x] - c.cigar_off[i0];
-            sh_src[threadIdx.x] = cig;
+    if (cig_on) {
+        if (t < m) {
+            sh_off[t] = co - range[4];
+            sh_src[t] = cig;
         }
         __syncthreads();
-        csr_fill_words(c.cigar, c.cigar_off, i0, m, sh_off, sh_src, st, a0, hi, b);
+        csr_fill_words(c.cigar, range[4], range[7], m, sh_off, sh_src, st, a0, hi, b);
         __syncthreads();
     }
-    if (c.seq && c.seq_off) {
-        if (threadIdx.x < m) {
-            sh_off[threadIdx.x] = c.seq_off[i0 + threadIdx.x] - c.seq_off[i0];
-            sh_src[threadIdx.x] = sq;
+    if (seq_on) {
+        if (t < m) {
+            sh_off[t] = so - range[5];
+            sh_src[t] = sq;
         }
         __syncthreads();
-        csr_fill_bytes(c.seq, c.seq_off, i0, m, sh_off, sh_src, st, a0, hi, b);
+        csr_fill_bytes(c.seq, range[5], range[8], m, sh_off, sh_src, st, a0, hi, b);
         __syncthreads();
     }
-    if (c.qual && c.qual_off) {
-        if (threadIdx.x < m) {
-            sh_off[threadIdx.x] = c.qual_off[i0 + threadIdx.x] - c.qual_off[i0];
-            sh_src[threadIdx.x] = ql;
+    if (qual_on) {
+        if (t < m) {
+            sh_off[t] = qo - range[6];
+            sh_src[t] = ql;
         }
         __syncthreads();
-        csr_fill_bytes(c.qual, c.qual_off, i0, m, sh_off, sh_src, st, a0, hi, b);
+        csr_fill_bytes(c.qual, range[6], range[9], m, sh_off, sh_src, st, a0, hi, b);
     }
 }
 
