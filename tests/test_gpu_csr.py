@@ -113,7 +113,8 @@ def test_chain_csr_of_fewer_records_than_the_chain(cbm, synth):
 
 def test_chain_csr_after_repair(cbm, oracle):
     """A walk that does not link is repaired (its records are not recorded): the CSR columns of
-    the repaired chain come from the serial walk of that segment."""
+    the repaired chain come from the serial walk of that segment. Fake records start segments 2
+    to 5, so that some of them start segments whose walk searches for its first record."""
     from cubam_amd import _decode, _lib
 
     L = _lib.load()
@@ -123,17 +124,18 @@ def test_chain_csr_after_repair(cbm, oracle):
     reads = bf.random_reads(300, seed=14)
     fake = b"".join(bf.Read(f"fake{i}", 0, 0, 100 + i, 60, [("M", 10)], seq="ACGTACGTAC").encode() for i in range(2))
     recs, o = [], 0
-    while o < seg - 3000:
+    while o < 2 * seg - 3000:
         r = reads[len(recs)].encode()
         recs.append(r)
         o += len(r)
     host = bf.Read("host", 0, 0, reads[len(recs)].pos, 60, [("M", 10)], seq="ACGTACGTAC")
     val = o + len(host.encode()) + 3 + 1 + 4
-    host.tags = [("XB", "B", ("C", list(b"\xff" * (seg - val) + fake + b"\xff" * 300)))]
+    arr = b"\xff" * (2 * seg - val) + (fake + b"\xff" * (seg - len(fake))) * 3 + fake + b"\xff" * 300
+    host.tags = [("XB", "B", ("C", list(arr)))]
     recs.append(host.encode())
     recs += [r.encode() for r in reads[len(recs):]]
     raw = np.frombuffer(b"".join(recs), np.uint8)
-    assert raw[seg : seg + len(fake)].tobytes() == fake
+    assert all(raw[k * seg : k * seg + len(fake)].tobytes() == fake for k in range(2, 6))
     data = _decode.to_device(raw)
     chain, off, csr, plain = _both(data, 0, int(raw.size), len(bf.REFS))
     assert chain.bad_links > 0 and len(off) == len(reads)

@@ -194,9 +194,10 @@ def test_spurious_bgzf_header_takes_the_serial_chain(cbm, oracle, tmp_path):
 
 
 def test_fake_records_at_a_segment_start_are_repaired(cbm, oracle, tmp_path):
-    """A B:C array holding two well-formed records, placed at the start of a walk segment: the
-    segment's walk starts on them, its links do not match, and the chain is repaired to the
-    oracle's."""
+    """A B:C array holding two well-formed records, placed at the starts of walk segments 2 to 5
+    (a wavefront walks consecutive segments and searches only in its first, so some of them are
+    searched): the segment's walk starts on them, its links do not match, and the chain is
+    repaired to the oracle's."""
     import gzip
 
     from cubam_amd import _decode, _lib
@@ -208,21 +209,23 @@ def test_fake_records_at_a_segment_start_are_repaired(cbm, oracle, tmp_path):
     reads = bf.random_reads(200, seed=12)
     fake = b"".join(bf.Read(f"fake{i}", 0, 0, 100 + i, 60, [("M", 10)], seq="ACGTACGTAC").encode() for i in range(2))
     recs, o = [], 0
-    while o < seg - 3000:  # ordinary records up to a little before the segment boundary
+    while o < 2 * seg - 3000:  # ordinary records up to a little before segment 2
         r = reads[len(recs)].encode()
         recs.append(r)
         o += len(r)
-    # a record whose B:C array covers the boundary, with the fake pair starting exactly on it
+    # a record whose B:C array covers segments 2 to 5, with the fake pair starting exactly on each
+    # of their starts
     host = bf.Read("host", 0, 0, reads[len(recs)].pos, 60, [("M", 10)], seq="ACGTACGTAC")
     head = len(host.encode())  # the record without aux
     val = o + head + 3 + 1 + 4  # where the array's values start
-    pad = seg - val
+    pad = 2 * seg - val
     assert 0 < pad < 4000
-    host.tags = [("XB", "B", ("C", list(b"\xff" * pad + fake + b"\xff" * 300)))]
+    arr = b"\xff" * pad + (fake + b"\xff" * (seg - len(fake))) * 3 + fake + b"\xff" * 300
+    host.tags = [("XB", "B", ("C", list(arr)))]
     recs.append(host.encode())
     recs += [r.encode() for r in reads[len(recs) :]]
     body = b"".join(recs)
-    assert body[seg : seg + len(fake)] == fake
+    assert all(body[k * seg : k * seg + len(fake)] == fake for k in range(2, 6))
     raw = bf.bgzf(bf.header_bytes() + body, 6000)
     hdr = len(bf.header_bytes())
     inflated = np.frombuffer(gzip.decompress(raw), np.uint8)
