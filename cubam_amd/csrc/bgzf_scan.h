@@ -16,15 +16,31 @@ namespace cbm {
 constexpr int BGZF_TILE = 16384;
 constexpr int BGZF_NT = 256;  // 64 bytes per thread: 4 vectors of 16
 
-// The fixed BGZF header fields at o, BSIZE + 1 >= 26, and the block inside [0, n)
-__device__ __forceinline__ bool bgzf_candidate(const uint8_t* b, int64_t n, int64_t o) {
+// The fixed BGZF header fields at o, BSIZE + 1 >= 26, and the block inside [0, n), read through an
+// accessor a (a.u8 at a data offset)
+template <class A>
+__device__ __forceinline__ bool bgzf_candidate_at(const A& a, int64_t n, int64_t o) {
     if (o + 18 > n) return false;
-    if (!(b[o] == 31 && b[o + 1] == 139 && b[o + 2] == 8 && b[o + 3] == 4 && b[o + 10] == 6 && b[o + 11] == 0 &&
-          b[o + 12] == 66 && b[o + 13] == 67 && b[o + 14] == 2 && b[o + 15] == 0))
+    if (!(a.u8(o) == 31 && a.u8(o + 1) == 139 && a.u8(o + 2) == 8 && a.u8(o + 3) == 4 && a.u8(o + 10) == 6 &&
+          a.u8(o + 11) == 0 && a.u8(o + 12) == 66 && a.u8(o + 13) == 67 && a.u8(o + 14) == 2 && a.u8(o + 15) == 0))
         return false;
-    int64_t bs = (int64_t)ld_u16(b, o + 16) + 1;
+    int64_t bs = (int64_t)(a.u8(o + 16) | (a.u8(o + 17) << 8)) + 1;
     return bs >= 26 && o + bs <= n;
 }
+__device__ __forceinline__ bool bgzf_candidate(const uint8_t* b, int64_t n, int64_t o) {
+    return bgzf_candidate_at(GlobalBytes{b}, n, o);
+}
+
+// A tile's bytes [base, base + len) staged in LDS, the rest read in place
+struct TileBytes {
+    const uint8_t* g;
+    const uint8_t* l;
+    int64_t base, len;
+    __device__ __forceinline__ uint32_t u8(int64_t o) const {
+        const int64_t r = o - base;
+        return (uint64_t)r < (uint64_t)len ? l[r] : g[o];
+    }
+};
 
 // A 16-byte mask of the bytes of a 4-word vector that equal 0x1f
 __device__ __forceinline__ uint32_t mask_1f(uint4 v) {
@@ -73,11 +89,38 @@ constexpr int BGZF_SLOTS = 8;
 
 // Candidates of each tile: their count to tile_cnt, and themselves, in order, to the tile's slots
 // when they fit there
+// A whole tile's candidates: its bytes are loaded four vectors a thread at once and kept in LDS,
+// where the header test of each 0x1f byte reads them (bytes past the tile are read in place)
+__device__ __forceinline__ uint64_t tile_candidates(const uint8_t* b, int64_t n, int64_t t0, uint8_t* tile) {
+    const int64_t o0 = t0 + (int64_t)threadIdx.x * 64;
+    uint4 q[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) q[v] = *(const uint4*)(b + o0 + 16 * v);  // buffers are 16-byte aligned at tile starts
+#pragma unroll
+    for (int v = 0; v < 4; v++) ((uint4*)tile)[threadIdx.x * 4 + v] = q[v];
+    __syncthreads();
+    const TileBytes a{b, tile, t0, BGZF_TILE};
+    uint64_t mask = 0;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        uint32_t m = mask_1f(q[v]);
+        while (m) {
+            const int j = __builtin_ctz(m);
+            m &= m - 1;
+            if (bgzf_candidate_at(a, n, o0 + 16 * v + j)) mask |= 1ull << (16 * v + j);
+        }
+    }
+    return mask;
+}
+
 __global__ void __launch_bounds__(BGZF_NT) bgzf_count(const uint8_t* b, int64_t n, int64_t* tile_cnt,
                                                       int64_t* slots) {
     __shared__ int64_t sh[BGZF_NT / CBM_WAVE];
-    int64_t o0 = (int64_t)blockIdx.x * BGZF_TILE + (int64_t)threadIdx.x * 64;
-    uint64_t m = thread_candidates(b, n, o0);
+    __shared__ __align__(16) uint8_t tile[BGZF_TILE];
+    const int64_t t0 = (int64_t)blockIdx.x * BGZF_TILE;
+    int64_t o0 = t0 + (int64_t)threadIdx.x * 64;
+    // a whole tile (all but the last, usually) through LDS; the last, partial one as before
+    uint64_t m = t0 + BGZF_TILE <= n ? tile_candidates(b, n, t0, tile) : thread_candidates(b, n, o0);
     int64_t tot;
     int64_t k = block_exclusive_sum<BGZF_NT, int64_t>((int64_t)__popcll(m), sh, &tot);
     if (tot <= BGZF_SLOTS) {

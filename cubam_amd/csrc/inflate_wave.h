@@ -613,20 +613,11 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
     } while (0)
     (void)stats;
 #endif
-    // this thread's CRC32 slice factor x^(8 * SLICE * (NT - 1 - tid)) mod P
+    // this thread's CRC32 slice factor x^(8 * SLICE * (NT - 1 - tid)) mod P, computed with the
+    // workgroup's first block (a launch over the blocks another kernel deferred often has none)
     constexpr int SLICE = (65536 + NT - 1) / NT;
-    uint32_t xfac;
-    {
-        uint32_t r = 0x80000000u, sq = 0x40000000u;  // x^0, x^1
-        for (int j = 0; j < 3; j++) sq = crc_mulmod(sq, sq);  // x^8
-        uint32_t k = (uint32_t)SLICE * (uint32_t)(NT - 1 - tid);
-        while (k) {
-            if (k & 1) r = crc_mulmod(r, sq);
-            sq = crc_mulmod(sq, sq);
-            k >>= 1;
-        }
-        xfac = r;
-    }
+    uint32_t xfac = 0;
+    bool have_xfac = false;
     __syncthreads();
 
     for (;;) {
@@ -634,6 +625,18 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
         __syncthreads();
         const int64_t kb = s.sv[SV_BLOCK];
         if (kb >= (n_dev ? *n_dev : n_blocks)) break;
+        if (!have_xfac) {
+            uint32_t r = 0x80000000u, sq = 0x40000000u;  // x^0, x^1
+            for (int j = 0; j < 3; j++) sq = crc_mulmod(sq, sq);  // x^8
+            uint32_t k = (uint32_t)SLICE * (uint32_t)(NT - 1 - tid);
+            while (k) {
+                if (k & 1) r = crc_mulmod(r, sq);
+                sq = crc_mulmod(sq, sq);
+                k >>= 1;
+            }
+            xfac = r;
+            have_xfac = true;
+        }
         const int64_t b = idx ? idx[kb] : kb;
         uint64_t* blist = TOK ? pool + (size_t)b * MCAP_TOK : nullptr;  // this block's match list (TOK)
         // this block's output, read once: a store through dst may alias out_off, so a read inside
