@@ -529,6 +529,44 @@ __device__ void stage(S& s, const uint8_t* in, int w0, int limit) {
     }
 }
 
+// n bytes from s to d in global memory (the two do not overlap), by the nt threads of a workgroup
+// (this one is t): bytes up to a 16-byte boundary of d, then 16-byte stores of d, each from five
+// aligned words of s realigned (four pieces a thread loaded before any is stored, so that the
+// loads are in flight together), then the last bytes. Reads up to 3 bytes before s and past
+// s + n, in the same buffer.
+__device__ __forceinline__ void copy_global(uint8_t* __restrict__ d, const uint8_t* __restrict__ s, int n, int t,
+                                            int nt) {
+    int head = (int)((16 - ((uintptr_t)d & 15)) & 15);
+    if (head > n) head = n;
+    for (int i = t; i < head; i += nt) d[i] = s[i];
+    d += head, s += head, n -= head;
+    const int nv = n >> 4;
+    const uint32_t sh = (uint32_t)((uintptr_t)s & 3);
+    const uint32_t* sw = (const uint32_t*)(s - sh);
+    for (int v0 = t; v0 < nv; v0 += 4 * nt) {
+        uint32_t a[4][5];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int v = v0 + j * nt < nv ? v0 + j * nt : v0;  // (past the last piece: this thread's first again)
+#pragma unroll
+            for (int k = 0; k < 5; k++) a[j][k] = sw[4 * v + k];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int v = v0 + j * nt;
+            if (v < nv) {
+                uint4 o;
+                o.x = __builtin_amdgcn_alignbyte(a[j][1], a[j][0], sh);
+                o.y = __builtin_amdgcn_alignbyte(a[j][2], a[j][1], sh);
+                o.z = __builtin_amdgcn_alignbyte(a[j][3], a[j][2], sh);
+                o.w = __builtin_amdgcn_alignbyte(a[j][4], a[j][3], sh);
+                *(uint4*)(d + 16 * v) = o;
+            }
+        }
+    }
+    for (int i = 16 * nv + t; i < n; i += nt) d[i] = s[i];
+}
+
 // ------------------------------------------------------------------ CRC32 tables
 // Slicing-by-4 tables (Kounavis and Berry, 2008): table k maps a byte followed by k zero bytes,
 // for the reflected polynomial 0xEDB88320. Built at compile time; read through the vector memory
@@ -708,8 +746,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
             if (type == 0) {
                 const int len = s.sv[SV_LEN], sp = s.sv[SV_SRC], op = s.sv[SV_OP];
                 if constexpr (TOK) {
-                    uint8_t* d = bdst + op;
-                    for (int i = tid; i < len; i += NT) d[i] = in[sp + i];
+                    copy_global(bdst + op, in + sp, len, tid, NT);
                 } else {
                     for (int i = tid; i < len; i += NT) s.out[op + i] = in[sp + i];
                 }
