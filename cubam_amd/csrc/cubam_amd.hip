@@ -188,7 +188,8 @@ static int launch_inflate_wave(const uint8_t* src, const int64_t* blk_off, const
                                const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
                                uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int64_t max_groups,
                                int64_t* next_block, uint64_t* mlists, int64_t* stats, const int64_t* idx,
-                               const int64_t* n_dev, uint64_t* pool, int32_t* mcount, hipStream_t st) {
+                               const int64_t* n_dev, uint64_t* pool, int32_t* mcount, hipStream_t st,
+                               int64_t mcap = MCAP_TOK, int64_t kmax = INT64_MAX) {
     const size_t lds = sizeof(WaveSmem<NT, SEG, TOK>);
     static bool attr = false;
     if (!attr) {
@@ -206,7 +207,7 @@ static int launch_inflate_wave(const uint8_t* src, const int64_t* blk_off, const
     if (g < 1) g = 1;
     hipLaunchKernelGGL((inflate_wave<NT, SEG, TOK, FUSE>), dim3((unsigned)g), dim3(NT), lds, st, src, blk_off, bsize, isize,
                        crc_expect, out_off, n_blocks, dst, crc, status, result, (unsigned long long*)next_block,
-                       mlists, (unsigned long long*)stats, idx, n_dev, pool, mcount);
+                       mlists, (unsigned long long*)stats, idx, n_dev, pool, mcount, mcap, kmax);
     return check_launch("inflate_wave");
 }
 
@@ -233,13 +234,14 @@ static int64_t tok_ws_bytes(int64_t n) { return ((12 * n + 255) / 256) * 256 + n
 template <int MPL, int SPAN, int WPG>
 static int launch_resolve(uint8_t* dst, const int64_t* out_off, const int32_t* isize, const uint32_t* crc_expect,
                           uint32_t* crc, int32_t* status, int64_t* result, const uint64_t* pool, const int32_t* mcount,
-                          int64_t n_blocks, hipStream_t st) {
+                          int64_t n_blocks, hipStream_t st, const int64_t* idx = nullptr,
+                          const int64_t* n_dev = nullptr, int64_t mcap = MCAP_TOK, int64_t kmax = INT64_MAX) {
     // the workgroups that fit a CU's LDS
     const int64_t per_cu = (160 * 1024) / (int64_t)((sizeof(ResSmem<SPAN, WPG>) + 511) / 512 * 512);
     int64_t g = (n_blocks + WPG - 1) / WPG;
     if (g > (int64_t)device_cus() * per_cu) g = (int64_t)device_cus() * per_cu;
     hipLaunchKernelGGL((inflate_resolve<MPL, SPAN, WPG>), dim3((unsigned)g), dim3(WPG * CBM_WAVE), 0, st, dst, out_off,
-                       isize, crc_expect, crc, status, result, pool, mcount, n_blocks);
+                       isize, crc_expect, crc, status, result, pool, mcount, n_blocks, idx, n_dev, mcap, kmax);
     return check_launch("inflate_resolve");
 }
 
@@ -363,8 +365,9 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
                       const uint32_t* crc_expect, const int64_t* out_off, int64_t n_blocks, uint8_t* dst,
                       uint32_t* crc, int32_t* status, int64_t* result, int64_t groups, int res_mpl, int res_wpg,
                       void* ws, int64_t ws_groups, int64_t max_groups, bool split, hipStream_t st) {
-    // [0] decode-phase next block (first part), [1] one-phase next block, [2] blocks left,
-    // [3] decode-phase next block of the second part
+    // [0] decode-phase next block (first part), [1] one-phase next block, [2] blocks the decode
+    // phase left, [3] decode-phase next block of the second part, [4] next block of the second
+    // pass over the blocks left, [5] blocks left after it
     int64_t* counter = (int64_t*)ws;
     uint64_t* mlists = (uint64_t*)((uint8_t*)ws + INFLATE_WS_HDR);
     uint8_t* tok = (uint8_t*)mlists + ws_groups * (int64_t)MLIST_CAP * 8;
@@ -403,9 +406,23 @@ static int launch_tok(const uint8_t* src, const int64_t* blk_off, const int32_t*
     if (lg > 1024) lg = 1024;
     hipLaunchKernelGGL(inflate_defer_list, dim3((unsigned)lg), dim3(256), 0, st, status, n_blocks, idx, counter + 2);
     if ((rc = check_launch("inflate_defer_list"))) return rc;
+    // the deferred blocks once more through both phases, each with a list of MLIST_CAP entries (as
+    // many matches as a block can hold) in the pool the first pass is done with: list kb for the
+    // kb-th deferred block, as many as the pool holds
+    const int64_t kmax2 = n_blocks * (int64_t)MCAP_TOK / MLIST_CAP;
+    if ((rc = launch_inflate_wave<64, SEG, true>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
+                                                 status, result, groups, (int64_t)1 << 40, counter + 4, nullptr,
+                                                 counter + 8, idx, counter + 2, pool, mcount, st, MLIST_CAP, kmax2)))
+        return rc;
+    if ((rc = launch_resolve<1, 2048, 4>(dst, out_off, isize, crc_expect, crc, status, result, pool, mcount,
+                                         n_blocks, st, idx, counter + 2, MLIST_CAP, kmax2)))
+        return rc;
+    // what is left (blocks that fail a check, and any past the pool) to the one-phase kernel
+    hipLaunchKernelGGL(inflate_defer_list, dim3((unsigned)lg), dim3(256), 0, st, status, n_blocks, idx, counter + 5);
+    if ((rc = check_launch("inflate_defer_list"))) return rc;
     return launch_inflate_wave<512, 128, false>(src, blk_off, bsize, isize, crc_expect, out_off, n_blocks, dst, crc,
                                                 status, result, 0, max_groups, counter + 1, mlists, counter + 8, idx,
-                                                counter + 2, nullptr, nullptr, st);
+                                                counter + 5, nullptr, nullptr, st);
 }
 
 // The chain's totals and per-segment bases: exclusive scans over the segments of the record

@@ -617,7 +617,8 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                                                    int32_t* status, int64_t* bad_count,
                                                    unsigned long long* next_block, uint64_t* mlists,
                                                    unsigned long long* stats, const int64_t* idx,
-                                                   const int64_t* n_dev, uint64_t* pool, int32_t* mcount) {
+                                                   const int64_t* n_dev, uint64_t* pool, int32_t* mcount,
+                                                   int64_t mcap, int64_t kmax) {
     using S = WaveSmem<NT, SEG, TOK>;
     static_assert(!FUSE || (TOK && NT == CBM_WAVE), "a fused resolve is one wavefront's");
     static_assert(!FUSE || S::IN_WORDS * 4 >= 2048 + 48 + ((2048 + 16) / 32 + 2) * 4,
@@ -625,6 +626,10 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
     extern __shared__ __align__(16) uint8_t smem_raw[];
     S& s = *(S*)smem_raw;
     const int tid = threadIdx.x;
+    if (n_dev) {  // a list of blocks: workgroups past its length have nothing to take (uniform)
+        const int64_t nd = *n_dev;
+        if ((int64_t)blockIdx.x >= (nd < kmax ? nd : kmax)) return;
+    }
     uint64_t* mlist = TOK ? nullptr : mlists + (size_t)blockIdx.x * MLIST_CAP;
 #if CBM_INFLATE_STATS
     // cycles per phase, kept by thread 0 and added to stats[] once per block
@@ -662,7 +667,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
         if (tid == 0) s.sv[SV_BLOCK] = (int)atomicAdd(next_block, 1ull);
         __syncthreads();
         const int64_t kb = s.sv[SV_BLOCK];
-        if (kb >= (n_dev ? *n_dev : n_blocks)) break;
+        if (kb >= (n_dev ? *n_dev : n_blocks) || kb >= kmax) break;
         if (!have_xfac) {
             uint32_t r = 0x80000000u, sq = 0x40000000u;  // x^0, x^1
             for (int j = 0; j < 3; j++) sq = crc_mulmod(sq, sq);  // x^8
@@ -676,7 +681,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
             have_xfac = true;
         }
         const int64_t b = idx ? idx[kb] : kb;
-        uint64_t* blist = TOK ? pool + (size_t)b * MCAP_TOK : nullptr;  // this block's match list (TOK)
+        uint64_t* blist = TOK ? pool + (size_t)kb * mcap : nullptr;  // this block's match list (TOK)
         // this block's output, read once: a store through dst may alias out_off, so a read inside
         // a store loop would be repeated (and waited for) at every store
         uint8_t* const bdst = dst + out_off[b];
@@ -1096,7 +1101,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                 const int total = (int)(uint32_t)tot2, mtotal = (int)(tot2 >> 32);
                 const int op = s.sv[SV_OP];
                 const int tstop = tend < NT ? s.stop_[tend] : STOP_NONE;
-                if (tstop == STOP_BAD || op + total > cap || (TOK && s.sv[SV_MCNT] + mtotal > MCAP_TOK)) {
+                if (tstop == STOP_BAD || op + total > cap || (TOK && s.sv[SV_MCNT] + mtotal > mcap)) {
                     __syncthreads();
                     if (tid == 0) {
                         s.sv[SV_ERR] = tstop == STOP_BAD ? CBM_BLOCK_DEFLATE : CBM_BLOCK_SIZE;
@@ -1661,17 +1666,22 @@ __global__ void __launch_bounds__(WPG * CBM_WAVE) inflate_resolve(uint8_t* dst, 
                                                                   const int32_t* isize, const uint32_t* expect,
                                                                   uint32_t* crc_out, int32_t* status, int64_t* bad,
                                                                   const uint64_t* pool, const int32_t* mcount,
-                                                                  int64_t n_blocks) {
+                                                                  int64_t n_blocks, const int64_t* idx,
+                                                                  const int64_t* n_dev, int64_t mcap, int64_t kmax) {
     static_assert(SPAN >= 258 && SPAN % 32 == 0, "a window holds one match at least");
     __shared__ ResSmem<SPAN, WPG> s;
+    // blocks idx[0 .. min(*n_dev, kmax)) with idx, else 0 .. min(n_blocks, kmax); list kb of the pool
+    const int64_t n = n_dev ? *n_dev : n_blocks, lim = n < kmax ? n : kmax;
+    if ((int64_t)blockIdx.x * WPG >= lim) return;  // (uniform) no block: skip the tables
     crc_tables(s.crc_t, s.crc_x);
     const int wv = (int)threadIdx.x / CBM_WAVE, lane = (int)threadIdx.x % CBM_WAVE;
     uint8_t* sp = s.span[wv];
     uint32_t* pend = s.pend[wv];
-    for (int64_t b = (int64_t)blockIdx.x * WPG + wv; b < n_blocks; b += (int64_t)gridDim.x * WPG) {
+    for (int64_t kb = (int64_t)blockIdx.x * WPG + wv; kb < lim; kb += (int64_t)gridDim.x * WPG) {
+        const int64_t b = idx ? idx[kb] : kb;
         if (status[b] != CBM_BLOCK_OK) continue;  // left to the one-phase kernel
         resolve_block<MPL, SPAN>(dst + out_off[b], isize[b], expect[b], crc_out + b, status + b, bad,
-                                 pool + (size_t)b * MCAP_TOK, mcount[b], sp, pend, s.crc_t,
+                                 pool + (size_t)kb * mcap, mcount[b], sp, pend, s.crc_t,
                                  s.crc_x[CBM_WAVE - 1 - lane], lane);
     }
 }
