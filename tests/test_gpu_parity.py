@@ -119,6 +119,31 @@ def test_split_inflates_on_two_streams_at_once(cbm):
         _say("two streams, rep", rep, "ok")
 
 
+@pytest.mark.parametrize("variant", [11, 15])
+def test_two_phase_second_pass_overflows_to_the_one_phase_kernel(cbm, variant):
+    """Every block has more matches than the first pass's list holds, so more blocks are deferred
+    than the second pass's full-size lists fit in the pool (about 28% of the blocks): the first
+    ones go through the second pass, the rest through the one-phase kernel, and every block comes
+    out as zlib makes it, CRC32 included."""
+    from cubam_amd import _lib
+
+    rng = random.Random(73)
+    vocab = [bytes(rng.choices(range(256), k=3)) for _ in range(6)]
+    payloads, blob = [], b""
+    for i in range(40):
+        n = 65280 - 97 * i
+        p = b"".join(rng.choice(vocab) + bytes([rng.randrange(256)]) for _ in range(16400))[:n]
+        payloads.append(p)
+        blob += bf.bgzf_block(p, 1, zlib.Z_DEFAULT_STRATEGY)
+    blob += bf.EOF_BLOCK
+    _say("second pass overflow, variant", variant)
+    blocks, inf = _gpu_blocks(np.frombuffer(blob, np.uint8), _lib.InflateTune(variant, 0, 0, 0))
+    assert int(inf.result.item()) == 0
+    assert (inf.status.cpu().numpy() == 0).all()
+    assert inf.data[: inf.n].cpu().numpy().tobytes() == b"".join(payloads)
+    assert inf.crc.cpu().numpy().astype(np.int64).tolist()[: len(payloads)] == [zlib.crc32(p) for p in payloads]
+
+
 def test_tiny_and_empty_blocks(cbm):
     # an empty block must be the 28-byte EOF block (src/cubam/bgzf.py:572), so b"" only at level 6
     payloads = [b"x", b"ab" * 3, bytes(range(256)), b"\0" * 35000, b"A" * 65280]
@@ -431,8 +456,9 @@ def test_corrupt_dynamic_headers_match_zlib(cbm, oracle, where, variant, bpw, wp
                                               (14, 0, 0), (15, 0, 0)])
 def test_two_phase_defers_blocks_it_cannot_finish(cbm, oracle, variant, bpw, wpg):
     """Blocks with more matches than the decode phase's list holds (3-byte matches all through),
-    stored blocks, fixed-Huffman blocks, and ordinary ones, mixed: the two-phase inflate leaves the
-    first kind to the one-phase kernel and every block comes out as zlib makes it, CRC32 included."""
+    stored blocks, fixed-Huffman blocks, and ordinary ones, mixed: the two-phase inflate takes the
+    first kind again in its second pass and every block comes out as zlib makes it, CRC32
+    included."""
     from cubam_amd import _lib
 
     rng = random.Random(71)
