@@ -101,7 +101,9 @@ enum {
     CBM_INFLATE_WG256 = 10,     /* NT 256, SEG 256 */
     /* two-phase: a wavefront decodes each block in SEG-bit segments, literals straight to dst_d and
        matches to a list; a resolve pass applies the matches in dst_d; then the CRC32 check. Blocks
-       the first phase cannot finish go through CBM_INFLATE_WG512_S128 (same results) */
+       with more matches than the first pass's lists hold go through both phases again with lists
+       of the most matches a block can have; blocks either pass cannot finish (any failed check)
+       go through CBM_INFLATE_WG512_S128 (same results) */
     CBM_INFLATE_TOK64_S512 = 11,
     CBM_INFLATE_TOK64_S1024 = 12,
     CBM_INFLATE_TOK64_S256 = 13,
