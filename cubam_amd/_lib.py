@@ -127,7 +127,7 @@ def load():
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            f"{LIB_PATH} is not built; run `python -m cubam_amd.build` "
             "(cubam_amd has no CPU fallback)"
         )
     L = ctypes.CDLL(str(LIB_PATH))
