@@ -186,11 +186,12 @@ struct Sym {
 
 // Decode tables of this variant. Literal/length entries are 32 bits:
 //   leaf:    bit 31 = 0, bits 0-3 = code length (0: no code), bits 4-6 = extra bits of a length,
-//            bits 8-9 = kind (0 literal, 1 length, 2 end of block, 3 symbol 286 or 287),
-//            bits 16-24 = the literal byte or the length's base (RFC 1951 3.2.5)
+//            bits 8-9 = kind (0 literal, 1 length, 2 end of block, 3 symbol 286 or 287, or no
+//            code), bits 16-24 = the literal byte or the length's base (RFC 1951 3.2.5)
 //   pointer: bit 31 = 1, bits 0-3 = index bits of the subtable, bits 4-14 = its offset
 // Distance entries are 16 bits:
-//   leaf:    bit 15 = 0, bits 0-3 = code length, bits 4-7 = extra bits, bits 8-12 = symbol
+//   leaf:    bit 15 = 0, bits 0-3 = code length (0: no code, and symbols 30 and 31, which only a
+//            fixed block can name), bits 4-7 = extra bits, bits 8-12 = symbol
 //   pointer: bit 15 = 1, bits 0-3 = index bits, bits 4-14 = offset
 // A subtable leaf holds the whole code length, root bits included. The code-length code's table
 // keeps bits 0-3 = length and bits 4-12 = symbol.
@@ -205,13 +206,15 @@ __device__ __forceinline__ uint32_t lit_entry(int sym, int len) {
     return ((uint32_t)base << 16) | (KIND_LEN << 8) | ((uint32_t)extra << 4) | (uint32_t)len;
 }
 __device__ __forceinline__ uint32_t dist_entry(int sym, int len) {
-    const int extra = sym < 4 ? 0 : (sym >> 1) - 1;  // symbols 30 and 31 are rejected by the decoder
+    if (sym >= 30) return 0;  // invalid distance symbols decode as no code
+    const int extra = sym < 4 ? 0 : (sym >> 1) - 1;
     return ((uint32_t)sym << 8) | ((uint32_t)extra << 4) | (uint32_t)len;
 }
 
 // decode one symbol (a literal/length code and, for a length, its distance) from the next
 // 64 bits of the stream; a symbol uses at most 15 + 5 + 15 + 13 = 48 bits. A code the table
-// does not hold, and distance symbols 30 and 31, give kind 3.
+// does not hold (distance symbols 30 and 31 included) gives kind 3; its bit count is then
+// meaningless (0 for a literal/length code), and every walk stops at such a symbol.
 template <class S>
 __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
     const uint32_t v0 = (uint32_t)v;
@@ -219,8 +222,8 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
     if ((int32_t)e < 0) e = s.lit[((e >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(v0, LBITS, e & 15)];
     const int cl = (int)(e & 15);
     Sym r;
-    r.kind = cl ? (int)((e >> 8) & 3) : (int)KIND_BAD;
-    r.bits = cl ? cl : 1;
+    r.kind = (int)((e >> 8) & 3);  // entries without a code hold KIND_BAD
+    r.bits = cl;
     r.val = (int)(e >> 16);
     r.dist = 0;
     if (r.kind == (int)KIND_LEN) {
@@ -230,9 +233,8 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
         uint32_t d = s.dist[x & ((1u << DBITS) - 1)];
         if (d & 0x8000) d = s.dist[((d >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(x, DBITS, d & 15)];
         const int dl = (int)(d & 15), de = (int)((d >> 4) & 15), ds = (int)((d >> 8) & 31);
-        const bool bad = !dl || ds >= 30;
-        r.kind = bad ? (int)KIND_BAD : (int)KIND_LEN;
-        r.bits = bad ? u + (dl ? dl : 1) : u + dl + de;
+        r.kind = dl ? (int)KIND_LEN : (int)KIND_BAD;
+        r.bits = u + dl + de;
         int db, de2;
         dist_code(ds, db, de2);
         r.dist = db + (int)__builtin_amdgcn_ubfe(x, dl, de);
@@ -240,15 +242,15 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
     return r;
 }
 
-// the stop a symbol causes: STOP_EOB at the end of block, STOP_BAD for a bad code or a symbol
-// that ends past the payload, else STOP_NONE
-__device__ __forceinline__ int sym_stop(const Sym& y, int p, int in_bits) {
-    const int k = y.kind >= 2 ? y.kind - 1 : STOP_NONE;
-    return k == STOP_NONE && p > in_bits ? STOP_BAD : k;
+// how a walk stopped, from its last symbol's kind and its position after it: STOP_EOB at the
+// end of block, STOP_BAD for a bad code or a symbol that ends past the payload, else STOP_NONE
+__device__ __forceinline__ int walk_stop(int kind, int p, int in_bits) {
+    return kind >= 2 ? kind - 1 : p > in_bits ? STOP_BAD : STOP_NONE;
 }
-// a symbol's output as a walk counts it: matches << 20 | bytes
+// a symbol's output as a walk counts it: matches << 20 | bytes (selects, not branches)
 __device__ __forceinline__ uint32_t sym_count(const Sym& y) {
-    return y.kind == 0 ? 1u : y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 0u;
+    const uint32_t c = y.kind == 0 ? 1u : (1u << 20) + (uint32_t)y.val;
+    return y.kind < 2 ? c : 0u;
 }
 
 // A walk's position on the staged stream. Each symbol is decoded from 64 bits read afresh from
@@ -303,8 +305,10 @@ __device__ bool coop_build(S& s, const uint8_t* lens, int nsym, int root, T* tab
     __syncthreads();
     if (!s.sv[SV_OK]) return false;
     const int max = s.sv[SV_MAX];
-    if (s.sv[SV_CLEAR])
-        for (int i = tid; i < rsize; i += NT) tab[i] = 0;
+    if (s.sv[SV_CLEAR]) {  // entries without a code: kind 3 in a literal/length table, length 0
+        const T none = (T)(type == TAB_LENS ? KIND_BAD << 8 : 0u);
+        for (int i = tid; i < rsize; i += NT) tab[i] = none;
+    }
     // canonical codes: rank of each symbol among symbols of its length, by 64-symbol chunks
     const int nchunk = (nsym + CBM_WAVE - 1) / CBM_WAVE;
     for (int c = wave; c < nchunk; c += NT / CBM_WAVE) {
@@ -999,12 +1003,16 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     }
                     // the segment, with a checkpoint every CP_STEPS symbols (the same step for all
                     // lanes of a wavefront, so recording one costs no test per symbol)
+                    // a walk goes on while its symbols are literals or matches and it is inside
+                    // both its segment and the payload; how it stopped is worked out once, after
+                    const int lim = min(aend, in_bits + 1);
+                    int lk = 0;  // the last symbol's kind
                     bool walking = w.p < aend;
                     auto step = [&]() {
                         const Sym y = win_next(w, s, wb);
                         run += sym_count(y);
-                        stop = sym_stop(y, w.p, in_bits);
-                        walking = stop == STOP_NONE && w.p < aend;
+                        lk = y.kind;
+                        walking = y.kind < 2 && w.p < lim;
                     };
 #pragma unroll
                     for (int j = 0; j < NCP; j++) {
@@ -1013,6 +1021,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     }
                     while (walking) step();
                     p = w.p;
+                    stop = walk_stop(lk, p, in_bits);
                 }
                 s.exit_[tid] = p;
                 s.stop_[tid] = active ? stop : STOP_BAD;
@@ -1042,7 +1051,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                         if (!__ballot(redo)) break;
                         if (redo) {
                             // walk from the new entry until a checkpoint of the first walk
-                            int nstop = STOP_NONE, sj = -1;
+                            int lk = 0, sj = -1;
                             uint32_t r2 = 0;
                             Win w;
                             win_at(w, s, want, wb);
@@ -1053,10 +1062,11 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                                 if (sj >= 0) break;
                                 Sym y = win_next(w, s, wb);
                                 r2 += sym_count(y);
-                                nstop = sym_stop(y, w.p, in_bits);
-                                if (nstop != STOP_NONE) break;
+                                lk = y.kind;
+                                if (y.kind >= 2 || w.p > in_bits) break;
                             }
                             int np = w.p;
+                            int nstop = walk_stop(lk, np, in_bits);
                             if (sj >= 0) {
                                 uint32_t c = 0;
 #pragma unroll
