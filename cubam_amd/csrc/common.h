@@ -124,6 +124,13 @@ __device__ inline bool is_record(const uint8_t* b, int64_t n, int64_t o, int n_r
 // ------------------------------------------------------------------ wave64 / workgroup scans
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// a value all lanes hold alike, in scalar registers (the first active lane's)
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_sum(T v) {
 #pragma unroll

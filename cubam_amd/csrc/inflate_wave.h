@@ -685,10 +685,12 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
             have_xfac = true;
         }
         const int64_t b = idx ? idx[kb] : kb;
-        uint64_t* blist = TOK ? pool + (size_t)kb * mcap : nullptr;  // this block's match list (TOK)
-        // this block's output, read once: a store through dst may alias out_off, so a read inside
-        // a store loop would be repeated (and waited for) at every store
-        uint8_t* const bdst = dst + out_off[b];
+        // this block's match list (TOK) and output. The output offset is read once (a store through
+        // dst may alias out_off, so a read inside a store loop would be repeated at every store),
+        // and both bases are made wave-uniform, so that pass C's stores take them from scalar
+        // registers with a 32-bit offset
+        uint64_t* blist = TOK ? pool + (size_t)uniform64(kb) * mcap : nullptr;
+        uint8_t* const bdst = dst + uniform64(out_off[b]);
         const uint8_t* in = src + blk_off[b] + 18;
         const int in_bytes = bsize[b] - 26;
         const int in_bits = 8 * in_bytes;
@@ -1128,22 +1130,25 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     bool bad = false;
                     Win w;
                     win_at(w, s, tid == 0 ? base : s.exit_[tid - 1], wb);
-                    while (w.p < e) {
+                    // the walk repeats a walk of pass A or of the links, which met no bad code and
+                    // stops after an end of block; a distance reaching before the block's first
+                    // byte marks the block bad (the stores stay inside its output and list)
+                    int kind = 0;
+                    while (w.p < e && kind < 2) {
                         const Sym y = win_next(w, s, wb);
-                        if (y.kind >= 2 || (y.kind == 1 && y.dist > o)) {
-                            bad = y.kind == 1;  // a distance reaching before the block's first byte
-                            break;
-                        }
-                        const uint64_t mrec = (uint64_t)(uint32_t)o | ((uint64_t)(uint32_t)y.dist << 17) |
-                                              ((uint64_t)(uint32_t)y.val << 33);
+                        kind = y.kind;
+                        bad |= kind == 1 && y.dist > o;
+                        // o | dist << 17 | len << 33 (a distance of 32,768 reaches bit 32)
+                        const uint64_t mrec = ((uint64_t)(((uint32_t)y.dist >> 15) | ((uint32_t)y.val << 1)) << 32) |
+                                              ((uint32_t)o | ((uint32_t)y.dist << 17));
                         if constexpr (TOK) {
-                            if (y.kind == 0) bdst[o] = (uint8_t)y.val;
-                            else blist[k++] = mrec;
+                            if (kind == 0) bdst[(uint32_t)o] = (uint8_t)y.val;
+                            else if (kind == 1) blist[(uint32_t)k++] = mrec;
                         } else {
-                            if (y.kind == 0) s.out[o] = (uint8_t)y.val;
-                            else mlist[k++] = mrec;
+                            if (kind == 0) s.out[o] = (uint8_t)y.val;
+                            else if (kind == 1) mlist[k++] = mrec;
                         }
-                        o += y.kind == 0 ? 1 : y.val;
+                        o += kind == 0 ? 1 : kind == 1 ? y.val : 0;
                     }
                     if (bad) {
                         atomicOr(&s.sv[SV_ERR], CBM_BLOCK_DEFLATE);

@@ -357,11 +357,6 @@ struct StagedBytes {
 // through block_size is followed 64 records at a time by the wavefront as one (scalar registers;
 // lane j keeps the j-th offset), then each lane tests its record; the first that fails ends the
 // walk there. The sums over the records (n_cigar_op, l_seq, odd l_seq) are DPP scans and ballots.
-__device__ __forceinline__ int64_t uniform64(int64_t x) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
 
 template <class A>
 __device__ WalkOut walk_wave(const A& a, int64_t n, int64_t o, int64_t hi, int n_ref, int lane, uint16_t* rel,
