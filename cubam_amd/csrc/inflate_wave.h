@@ -141,6 +141,23 @@ __device__ __forceinline__ uint64_t peek64(const S& s, int p, int wbase) {
     return ((uint64_t)__builtin_amdgcn_alignbit(hi, mid, sh) << 32) | __builtin_amdgcn_alignbit(mid, lo, sh);
 }
 
+// The LDS byte address of a round's staging area less 4 * wbase (wave-uniform), so that the staged
+// word holding bit p sits at sb + 4 * (p >> 5): a walk's step then finds its words with one shift
+// and one add of a scalar
+template <class S>
+__device__ __forceinline__ uint32_t stage_base(const S& s, int wbase) {
+    return (uint32_t)(uintptr_t)(const CBM_LDS uint32_t*)s.in - 4u * (uint32_t)wbase;
+}
+// peek64 through stage_base's address
+__device__ __forceinline__ uint64_t peek64_sb(uint32_t sb, int p) {
+    // the word index by a bit-field extract, which the compiler keeps (a shift it would fold with the
+    // scaling into a shift, a mask and an add), then one shift-and-add
+    const CBM_LDS uint32_t* q = (const CBM_LDS uint32_t*)(uintptr_t)(sb + __builtin_amdgcn_ubfe((uint32_t)p, 5, 27) * 4u);
+    const uint32_t lo = q[0], mid = q[1], hi = q[2];
+    const uint32_t sh = (uint32_t)p & 31;
+    return ((uint64_t)__builtin_amdgcn_alignbit(hi, mid, sh) << 32) | __builtin_amdgcn_alignbit(mid, lo, sh);
+}
+
 // 32 bits of the staged stream from bit p
 template <class S>
 __device__ __forceinline__ uint32_t peek32(const S& s, int p, int wbase) {
@@ -247,11 +264,13 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
 __device__ __forceinline__ int walk_stop(int kind, int p, int in_bits) {
     return kind >= 2 ? kind - 1 : p > in_bits ? STOP_BAD : STOP_NONE;
 }
-// a symbol's output as a walk counts it: matches << 20 | bytes (selects, not branches)
+// a symbol's output as a walk counts it: matches << 20 | bytes. The symbol that stops a walk (an end
+// of block or a bad code) counts as a byte here; the walk takes it back once it has stopped
+// (stop_count), which saves a select every step
 __device__ __forceinline__ uint32_t sym_count(const Sym& y) {
-    const uint32_t c = y.kind == 0 ? 1u : (1u << 20) + (uint32_t)y.val;
-    return y.kind < 2 ? c : 0u;
+    return y.kind == 1 ? (1u << 20) + (uint32_t)y.val : 1u;
 }
+__device__ __forceinline__ uint32_t stop_count(int last_kind) { return last_kind >= 2 ? 1u : 0u; }
 
 // A walk's position on the staged stream. Each symbol is decoded from 64 bits read afresh from
 // LDS at the position (three words, two funnel shifts): cheaper than keeping a register window
@@ -264,8 +283,8 @@ __device__ __forceinline__ void win_at(Win& w, const S&, int p, int) {
     w.p = p;
 }
 template <class S>
-__device__ __forceinline__ Sym win_next(Win& w, const S& s, int wb) {
-    const Sym y = decode_v(s, peek64(s, w.p, wb));
+__device__ __forceinline__ Sym win_next(Win& w, const S& s, uint32_t sb) {
+    const Sym y = decode_v(s, peek64_sb(sb, w.p));
     w.p += y.bits;
     return y;
 }
@@ -960,6 +979,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
             for (;;) {
                 const int base = s.sv[SV_POS];
                 const int wb = base >> 5;
+                const uint32_t sb = stage_base(s, wb);
                 stage<NT>(s, in, wb, limit);
                 __syncthreads();
                 // segment bits this round: SEG, or fewer in the last round of the final Huffman
@@ -997,7 +1017,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     win_at(w, s, a - OVL > base ? a - OVL : base, wb);
                     // the overlap: symbol lengths only, up to the first symbol start at or after a
                     while (w.p < a) {
-                        const Sym y = win_next(w, s, wb);
+                        const Sym y = win_next(w, s, sb);
                         if (y.kind >= 2 || w.p > in_bits) {
                             win_at(w, s, a, wb);
                             break;
@@ -1011,7 +1031,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     int lk = 0;  // the last symbol's kind
                     bool walking = w.p < aend;
                     auto step = [&]() {
-                        const Sym y = win_next(w, s, wb);
+                        const Sym y = win_next(w, s, sb);
                         run += sym_count(y);
                         lk = y.kind;
                         walking = y.kind < 2 && w.p < lim;
@@ -1024,6 +1044,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     while (walking) step();
                     p = w.p;
                     stop = walk_stop(lk, p, in_bits);
+                    run -= stop_count(lk);
                 }
                 s.exit_[tid] = p;
                 s.stop_[tid] = active ? stop : STOP_BAD;
@@ -1062,13 +1083,14 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                                 for (int j = 0; j < NCP; j++)
                                     if (w.p == cp[j]) sj = j;
                                 if (sj >= 0) break;
-                                Sym y = win_next(w, s, wb);
+                                Sym y = win_next(w, s, sb);
                                 r2 += sym_count(y);
                                 lk = y.kind;
                                 if (y.kind >= 2 || w.p > in_bits) break;
                             }
                             int np = w.p;
                             int nstop = walk_stop(lk, np, in_bits);
+                            r2 -= stop_count(lk);
                             if (sj >= 0) {
                                 uint32_t c = 0;
 #pragma unroll
@@ -1135,7 +1157,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     // byte marks the block bad (the stores stay inside its output and list)
                     int kind = 0;
                     while (w.p < e && kind < 2) {
-                        const Sym y = win_next(w, s, wb);
+                        const Sym y = win_next(w, s, sb);
                         kind = y.kind;
                         bad |= kind == 1 && y.dist > o;
                         // o | dist << 17 | len << 33 (a distance of 32,768 reaches bit 32)
