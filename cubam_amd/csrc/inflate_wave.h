@@ -44,6 +44,11 @@ enum { SV_POS = 0, SV_FINAL, SV_TYPE, SV_ERR, SV_LEN, SV_SRC, SV_OP, SV_CLEAR, S
 template <int NT>
 constexpr int md_k() { return NT >= 512 ? 4 : 8; }
 
+// The distance table of these kernels: an 8-bit root takes nearly every distance code in one
+// lookup, and its worst case (30 symbols, codes of at most 15 bits, in canonical order) is 400
+// entries, fewer than the 592 a 6-bit root can need (zlib's bound for its 6-bit root)
+constexpr int WDBITS = 8, WDTAB = 400;
+
 // TOK: the decode phase of the two-phase inflate (tokens to global memory, no output window in LDS)
 template <int NT, int SEG, bool TOK = false>
 struct WaveSmem {
@@ -53,7 +58,7 @@ struct WaveSmem {
     // the tables and the staging area first: their offsets fit the 16-bit offset field of the
     // LDS instructions, so a lookup needs no address add
     uint32_t lit[LTAB];
-    uint16_t dist[DTAB];
+    uint16_t dist[WDTAB];
     uint32_t in[IN_WORDS];
     union {
         struct {  // header and table build
@@ -109,7 +114,7 @@ static_assert(8 * (8 + STAT_N) <= INFLATE_WS_HDR, "statistics overrun the worksp
 // most 21,846 matches of 3 bytes or more
 constexpr int MLIST_CAP = 21888;
 
-// length / distance bases and extra bits of RFC 1951 3.2.5, computed
+// length bases and extra bits of RFC 1951 3.2.5, computed
 __device__ __forceinline__ void len_code(int c, int& base, int& extra) {
     if (c < 8) {
         base = c + 3;
@@ -120,15 +125,6 @@ __device__ __forceinline__ void len_code(int c, int& base, int& extra) {
     } else {
         base = 258;
         extra = 0;
-    }
-}
-__device__ __forceinline__ void dist_code(int d, int& base, int& extra) {
-    if (d < 4) {
-        base = d + 1;
-        extra = 0;
-    } else {
-        extra = (d >> 1) - 1;
-        base = ((2 + (d & 1)) << extra) + 1;
     }
 }
 
@@ -208,7 +204,8 @@ struct Sym {
 //   pointer: bit 31 = 1, bits 0-3 = index bits of the subtable, bits 4-14 = its offset
 // Distance entries are 16 bits:
 //   leaf:    bit 15 = 0, bits 0-3 = code length (0: no code, and symbols 30 and 31, which only a
-//            fixed block can name), bits 4-7 = extra bits, bits 8-12 = symbol
+//            fixed block can name), bits 4-7 = extra bits e, bits 8-9 = m, the distance's base
+//            being (m << e) + 1 (RFC 1951 3.2.5: m = the symbol below 4, else 2 + its low bit)
 //   pointer: bit 15 = 1, bits 0-3 = index bits, bits 4-14 = offset
 // A subtable leaf holds the whole code length, root bits included. The code-length code's table
 // keeps bits 0-3 = length and bits 4-12 = symbol.
@@ -224,8 +221,8 @@ __device__ __forceinline__ uint32_t lit_entry(int sym, int len) {
 }
 __device__ __forceinline__ uint32_t dist_entry(int sym, int len) {
     if (sym >= 30) return 0;  // invalid distance symbols decode as no code
-    const int extra = sym < 4 ? 0 : (sym >> 1) - 1;
-    return ((uint32_t)sym << 8) | ((uint32_t)extra << 4) | (uint32_t)len;
+    const int extra = sym < 4 ? 0 : (sym >> 1) - 1, m = sym < 4 ? sym : 2 | (sym & 1);
+    return ((uint32_t)m << 8) | ((uint32_t)extra << 4) | (uint32_t)len;
 }
 
 // decode one symbol (a literal/length code and, for a length, its distance) from the next
@@ -247,14 +244,12 @@ __device__ __forceinline__ Sym decode_v(const S& s, uint64_t v) {
         const int ex = (int)((e >> 4) & 7), u = cl + ex;
         r.val += (int)__builtin_amdgcn_ubfe(v0, cl, ex);
         const uint32_t x = (uint32_t)(v >> u);
-        uint32_t d = s.dist[x & ((1u << DBITS) - 1)];
-        if (d & 0x8000) d = s.dist[((d >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(x, DBITS, d & 15)];
-        const int dl = (int)(d & 15), de = (int)((d >> 4) & 15), ds = (int)((d >> 8) & 31);
+        uint32_t d = s.dist[x & ((1u << WDBITS) - 1)];
+        if (d & 0x8000) d = s.dist[((d >> 4) & 0x7ff) + __builtin_amdgcn_ubfe(x, WDBITS, d & 15)];
+        const int dl = (int)(d & 15), de = (int)((d >> 4) & 15), dm = (int)((d >> 8) & 3);
         r.kind = dl ? (int)KIND_LEN : (int)KIND_BAD;
         r.bits = u + dl + de;
-        int db, de2;
-        dist_code(ds, db, de2);
-        r.dist = db + (int)__builtin_amdgcn_ubfe(x, dl, de);
+        r.dist = (dm << de) + 1 + (int)__builtin_amdgcn_ubfe(x, dl, de);
     }
     return r;
 }
@@ -791,7 +786,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                 coop_build<NT>(s, s.lens, 288, LBITS, s.lit, LTAB, TAB_LENS, s.in);
                 for (int i = tid; i < 32; i += NT) s.lens[i] = 5;
                 __syncthreads();
-                coop_build<NT>(s, s.lens, 32, DBITS, s.dist, DTAB, TAB_DISTS, s.in);
+                coop_build<NT>(s, s.lens, 32, WDBITS, s.dist, WDTAB, TAB_DISTS, s.in);
             } else {
                 const int hlit = s.sv[SV_LEN], hdist = s.sv[SV_SRC];
                 if (tid < 19) {
@@ -801,7 +796,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                 __syncthreads();
                 // the code-length code, then the code lengths (wavefront 0); the code-length
                 // table goes in the distance table's space
-                bool ok = coop_build<NT>(s, s.lens, 19, CBITS, s.dist, DTAB, TAB_CODES, s.in);
+                bool ok = coop_build<NT>(s, s.lens, 19, CBITS, s.dist, WDTAB, TAB_CODES, s.in);
                 if (!ok) {
                     if (tid == 0) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_CLCODE;
                     __syncthreads();
@@ -966,7 +961,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                     __syncthreads();
                     for (int i = tid; i < hdist; i += NT) s.lens[i] = (uint8_t)s.code[i];
                     __syncthreads();
-                    ok = coop_build<NT>(s, s.lens, hdist, DBITS, s.dist, DTAB, TAB_DISTS, s.in);
+                    ok = coop_build<NT>(s, s.lens, hdist, WDBITS, s.dist, WDTAB, TAB_DISTS, s.in);
                 }
                 if (!ok) {
                     if (tid == 0) s.sv[SV_ERR] = CBM_BLOCK_DEFLATE, s.sv[SV_SITE] = SITE_TABLES;
@@ -1170,7 +1165,7 @@ __global__ void __launch_bounds__(NT, (TOK ? 4 : NT >= 256 ? NT / 128 : 1)) infl
                             if (kind == 0) s.out[o] = (uint8_t)y.val;
                             else if (kind == 1) mlist[k++] = mrec;
                         }
-                        o += kind == 0 ? 1 : kind == 1 ? y.val : 0;
+                        o += kind == 1 ? y.val : 1;  // (past an end of block, o is not used)
                     }
                     if (bad) {
                         atomicOr(&s.sv[SV_ERR], CBM_BLOCK_DEFLATE);
