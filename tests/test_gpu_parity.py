@@ -394,6 +394,114 @@ def test_crc32_spans_matches_zlib(cbm):
     assert got == [zlib.crc32(raw[o : o + s].tobytes()) for o, s in zip(off, size)]
 
 
+class _Bits:
+    """A DEFLATE bit stream: integers least significant bit first, Huffman codes most significant
+    bit first (RFC 1951 3.1.1)."""
+
+    def __init__(self):
+        self.v, self.n = 0, 0
+
+    def put(self, value, nbits):
+        self.v |= value << self.n
+        self.n += nbits
+
+    def huff(self, code, nbits):
+        self.put(int(format(code, f"0{nbits}b")[::-1], 2), nbits)
+
+    def data(self):
+        return self.v.to_bytes((self.n + 7) // 8, "little")
+
+
+def _fixed_litlen(b, sym):
+    """A literal/length symbol in the fixed Huffman code (RFC 1951 3.2.6)."""
+    if sym < 144:
+        b.huff(0x30 + sym, 8)
+    elif sym < 256:
+        b.huff(0x190 + sym - 144, 9)
+    elif sym < 280:
+        b.huff(sym - 256, 7)
+    else:
+        b.huff(0xC0 + sym - 280, 8)
+
+
+# RFC 1951 3.2.5: length and distance bases and extra bits, symbol by symbol
+_LEN_BASE = [3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31, 35, 43, 51, 59, 67, 83, 99, 115, 131,
+             163, 195, 227, 258]
+_LEN_EXTRA = [0] * 8 + [1] * 4 + [2] * 4 + [3] * 4 + [4] * 4 + [5] * 4 + [0]
+_DIST_BASE = [1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97, 129, 193, 257, 385, 513, 769, 1025, 1537,
+              2049, 3073, 4097, 6145, 8193, 12289, 16385, 24577]
+_DIST_EXTRA = [0, 0, 0, 0] + [k // 2 for k in range(2, 28)]
+
+
+def _fixed_block(seed, bad=None):
+    """One final fixed-Huffman block: 32,768 random literals, a match with every distance symbol
+    (0-29) and every length symbol (257-285), each with random extra bits, then end of block.
+    bad = ("dist", 30 or 31) or ("litlen", 286 or 287) puts that symbol in the stream instead,
+    a few literals in."""
+    rng = random.Random(seed)
+    b = _Bits()
+    b.put(1, 1)  # BFINAL
+    b.put(1, 2)  # BTYPE 01: fixed Huffman codes
+    nlit = 40 if bad else 32768
+    for _ in range(nlit):
+        _fixed_litlen(b, rng.randrange(256))
+    if bad and bad[0] == "litlen":
+        _fixed_litlen(b, bad[1])
+    elif bad:
+        _fixed_litlen(b, 257)
+        b.huff(bad[1], 5)
+    else:
+        pairs = [(rng.randrange(29), d) for d in range(30)] + [(c, rng.randrange(8)) for c in range(29)]
+        rng.shuffle(pairs)
+        for lc, dc in pairs:
+            _fixed_litlen(b, 257 + lc)
+            b.put(rng.randrange(1 << _LEN_EXTRA[lc]), _LEN_EXTRA[lc])
+            b.huff(dc, 5)
+            b.put(rng.randrange(1 << _DIST_EXTRA[dc]), _DIST_EXTRA[dc])
+    _fixed_litlen(b, 256)
+    return b.data()
+
+
+@pytest.mark.parametrize("variant", [0, 11, 15])
+def test_fixed_blocks_every_length_and_distance_code_match_zlib(cbm, variant):
+    """Fixed-Huffman blocks written bit by bit: every length symbol and every distance symbol
+    0-29 with random extra bits (the largest distances reach back 32 KiB), against zlib's bytes;
+    and blocks naming distance symbols 30 or 31 or literal/length symbols 286 or 287, which zlib
+    rejects ("invalid distance code", "invalid literal/length code") and which must be rejected
+    here."""
+    from cubam_amd import _lib
+
+    cases = [(s, None) for s in range(3)] + [(7, ("dist", 30)), (8, ("dist", 31)), (9, ("litlen", 286)),
+                                             (10, ("litlen", 287))]
+    blob, want = bytearray(), []
+    for seed, bad in cases:
+        comp = _fixed_block(seed, bad)
+        try:
+            p = zlib.decompressobj(-15).decompress(comp)
+        except zlib.error:
+            p = None
+        assert (p is None) == (bad is not None), (seed, bad)
+        payload = p if p is not None else bytes(64)
+        head = b"\x1f\x8b\x08\x04\x00\x00\x00\x00\x00\xff\x06\x00BC\x02\x00" + (18 + len(comp) + 7).to_bytes(2, "little")
+        blob += head + comp + (zlib.crc32(payload) & 0xFFFFFFFF).to_bytes(4, "little") + len(payload).to_bytes(4, "little")
+        want.append(p)
+    blob += bf.EOF_BLOCK
+    _say("fixed blocks: variant", variant, len(want), "blocks")
+    blocks, inf = _gpu_blocks(np.frombuffer(bytes(blob), np.uint8), _lib.InflateTune(variant, 0, 0, 0))
+    assert blocks.n_blocks == len(want) + 1
+    st = inf.status.cpu().numpy()[: len(want)] & 255
+    data = inf.data[: inf.n].cpu().numpy()
+    o = 0
+    for k, p in enumerate(want):
+        if p is None:
+            assert st[k] == _lib.BLOCK_DEFLATE, (k, cases[k], st[k])
+            o += 64
+        else:
+            assert st[k] == _lib.BLOCK_OK, (k, st[k])
+            assert data[o : o + len(p)].tobytes() == p, k
+            o += len(p)
+
+
 @pytest.mark.parametrize("variant,bpw,wpg", [(0, 0, 0), (11, 1, 4), (11, 4, 2), (11, 8, 4), (11, 16, 2),
                                               (14, 0, 0), (15, 0, 0)])
 @pytest.mark.parametrize("where", ["header", "anywhere"])
