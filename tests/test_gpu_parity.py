@@ -462,6 +462,89 @@ def _fixed_block(seed, bad=None):
     return b.data()
 
 
+def _canonical(lengths):
+    """Canonical Huffman codes of the given code lengths (RFC 1951 3.2.2), 0 for unused symbols."""
+    count = [0] * 16
+    for n in lengths:
+        count[n] += 1
+    count[0] = 0
+    code, nxt = 0, [0] * 16
+    for n in range(1, 16):
+        code = (code + count[n - 1]) << 1
+        nxt[n] = code
+    out = []
+    for n in lengths:
+        out.append(nxt[n] if n else 0)
+        if n:
+            nxt[n] += 1
+    return out
+
+
+def _deep_dynamic_block(seed):
+    """One final dynamic-Huffman block whose distance code is a chain: 16 of the 30 distance symbols
+    (shuffled) take code lengths 1, 2, ..., 14, 15, 15, so the codes of 9 to 15 bits share one
+    8-bit root prefix and a 128-entry subtable. Literal/length lengths are 8 and 9 bits; the
+    code-length code gives each length 0-15 four bits. Data: 32,768 random literals, then matches
+    with every used distance symbol and random length symbols and extra bits, then end of block."""
+    rng = random.Random(seed)
+    lit = [8] * 286
+    for sym in range(196, 256):
+        lit[sym] = 9  # 226 / 256 + 60 / 512 = 1
+    dsyms = rng.sample(range(30), 16)
+    dist = [0] * 30
+    for k, sym in enumerate(dsyms):
+        dist[sym] = min(k + 1, 15)
+    lcode, dcode = _canonical(lit), _canonical(dist)
+    b = _Bits()
+    b.put(1, 1)  # BFINAL
+    b.put(2, 2)  # BTYPE 10: dynamic Huffman codes
+    b.put(286 - 257, 5)
+    b.put(30 - 1, 5)
+    b.put(19 - 4, 4)
+    for sym in (16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15):
+        b.put(0 if sym > 15 else 4, 3)
+    for n in lit + dist:
+        b.huff(n, 4)  # sixteen 4-bit codes: each length's code is its value
+    for _ in range(32768):
+        sym = rng.randrange(256)
+        b.huff(lcode[sym], lit[sym])
+    pairs = [(rng.randrange(29), d) for d in dsyms for _ in range(3)]
+    rng.shuffle(pairs)
+    for lc, dc in pairs:
+        b.huff(lcode[257 + lc], lit[257 + lc])
+        b.put(rng.randrange(1 << _LEN_EXTRA[lc]), _LEN_EXTRA[lc])
+        b.huff(dcode[dc], dist[dc])
+        b.put(rng.randrange(1 << _DIST_EXTRA[dc]), _DIST_EXTRA[dc])
+    b.huff(lcode[256], lit[256])
+    return b.data()
+
+
+@pytest.mark.parametrize("variant", [0, 11, 15])
+def test_deep_distance_codes_match_zlib(cbm, variant):
+    """Dynamic-Huffman blocks whose distance codes run to 15 bits (a subtable under the 8-bit
+    distance root), against zlib's bytes."""
+    from cubam_amd import _lib
+
+    blob, want = bytearray(), []
+    for seed in range(4):
+        comp = _deep_dynamic_block(seed)
+        p = zlib.decompressobj(-15).decompress(comp)
+        head = b"\x1f\x8b\x08\x04\x00\x00\x00\x00\x00\xff\x06\x00BC\x02\x00" + (18 + len(comp) + 7).to_bytes(2, "little")
+        blob += head + comp + (zlib.crc32(p) & 0xFFFFFFFF).to_bytes(4, "little") + len(p).to_bytes(4, "little")
+        want.append(p)
+    blob += bf.EOF_BLOCK
+    _say("deep distance codes: variant", variant, len(want), "blocks")
+    blocks, inf = _gpu_blocks(np.frombuffer(bytes(blob), np.uint8), _lib.InflateTune(variant, 0, 0, 0))
+    assert blocks.n_blocks == len(want) + 1
+    st = inf.status.cpu().numpy()[: len(want)] & 255
+    data = inf.data[: inf.n].cpu().numpy()
+    o = 0
+    for k, p in enumerate(want):
+        assert st[k] == _lib.BLOCK_OK, (k, st[k])
+        assert data[o : o + len(p)].tobytes() == p, k
+        o += len(p)
+
+
 @pytest.mark.parametrize("variant", [0, 11, 15])
 def test_fixed_blocks_every_length_and_distance_code_match_zlib(cbm, variant):
     """Fixed-Huffman blocks written bit by bit: every length symbol and every distance symbol
